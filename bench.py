@@ -1,0 +1,274 @@
+"""Benchmark: extrapolated points/s of the WLSQ extrapolate-and-image path.
+
+Workload (one GPU): 512 x 1024 model at 10 m, 16 shots, nf = 256 over
+2-50 Hz, 31-tap operators on 2048 kappa samples (complex64, FP32 math).
+One step = all 16 shots through every depth step, both wavefields plus the
+imaging condition, i.e. 16 * 512 * 1024 * 256 = 2.147e9 points.
+
+* value ("batched"): the fused compact-table kernel - one launch for all 16
+  shots, operators gathered on the device by the kappa lookup, wavefield rows
+  resident in LDS for all depth steps, image accumulated on the device.
+* plain: the drop-in path on the same inputs - expanded tap-major tables
+  (2 x 33.3 GB) resident in HBM, one launch sequence per shot (one launch per
+  depth step per shot), fixed-order image reduction.
+* cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference CPU
+  path, compact-table mode) on a bounded sample on this host.
+
+Inputs are resident in HBM before the timed region. Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "extrapolated points/s (nx·nz·nf·shots) batched vs plain; p50/p99 job latency"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_per_point(L):
+    return 16 * L + 4
+
+
+def plain_bytes_per_point(L):
+    return 16 * L + 32
+
+
+def read_profile_traffic(name):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 summary
+    (profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(p, nshots_sample=2):
+    """CPU oracle (port of the reference CPU path) on `nshots_sample` shots of
+    the workload, all depth steps, every frequency."""
+    import numpy as np
+
+    cores = len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    import oracle
+    from imaging2d_wlsq_amd import synth
+
+    oracle.lib()
+    t, m = p.tables, p.model
+    shots = p.shots[:nshots_sample]
+    F, B = synth.host_fields(p, shots)
+    s = shots[0]
+    img = np.zeros((len(shots), m.nz, m.nx), np.float32)
+    t0 = time.perf_counter()
+    oracle.extrapolate_ktable(len(shots), m.nz, m.nz, m.nx, s.nf, s.nf, t.M, m.slowness, s.omega,
+                              t.kappa0, t.dkappa, t.nk, t.forw, t.back, F, B, img)
+    dt = time.perf_counter() - t0
+    pts = len(shots) * m.nz * m.nx * s.nf
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": pts / dt, "unit": "points/s", "cores": cores, "kind": "port",
+            "sample": f"{len(shots)} of {len(p.shots)} shots, all {m.nz} depth steps, nf {s.nf}, "
+                      f"nx {m.nx} ({pts:.3g} points, {dt:.1f} s, compact-table mode, {cpu})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shots-per-group", type=int, default=0)
+    ap.add_argument("--omegas-per-item", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--x-per-thread", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
+              "shot across GPUs with one image reduction at the end, which is not built here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from imaging2d_wlsq_amd import device, synth
+    from imaging2d_wlsq_amd._lib import LAYOUT_TAPMAJOR, lib
+
+    lib()
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log("building workload ...")
+    p = synth.config_c2()
+    t, m = p.tables, p.model
+    L = t.L
+    ns, nz, nx = len(p.shots), m.nz, m.nx
+    nf = p.shots[0].nf
+    points = p.points
+    slow = torch.from_numpy(m.slowness).to(dev)
+    omega = torch.from_numpy(p.shots[0].omega).to(dev)
+    tf = torch.from_numpy(t.forw).to(dev)
+    tb = torch.from_numpy(t.back).to(dev)
+    conj = device.tables_conjugate(tf, tb)
+    F0, B0, offs = synth.device_fields(p, p.shots, device=dev)
+    F0 = F0.view(ns, nf, nx)
+    B0 = B0.view(ns, nf, nx)
+    stream = torch.cuda.current_stream()
+
+    # ---------------------------------------------------------------- batched
+    specs = [device.ShotSpec(f_off=s * nf * nx, b_off=s * nf * nx, img_off=0, slow_off=0, ld=nx,
+                             img_ld=nx, slow_ld=nx, nx=nx, nz=nz, nf=nf, omega_off=0)
+             for s in range(ns)]
+    tuning = dict(tables_conjugate=int(conj), shots_per_group=args.shots_per_group,
+                  omegas_per_item=args.omegas_per_item, block=args.block,
+                  x_per_thread=args.x_per_thread)
+    plan = device.Plan(specs, t.M, tuning)
+    info = plan.info()
+    log("plan", info)
+    F = F0.clone()
+    B = B0.clone()
+    image = torch.zeros((nz, nx), dtype=torch.float32, device=dev)
+
+    def fused_step():
+        image.zero_()
+        plan.launch(slow, omega, t.kappa0, t.dkappa, t.nk, tf, tb, F, B, image, write_back=False,
+                    stream=stream)
+
+    for _ in range(args.warmup):
+        fused_step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        image.zero_()
+        e0.record(stream)
+        plan.launch(slow, omega, t.kappa0, t.dkappa, t.nk, tf, tb, F, B, image, write_back=False,
+                    stream=stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    k_avg = sum(kernel_ms) / len(kernel_ms)
+    ms_per_step = wall * 1e3 / args.steps
+    value = points / (ms_per_step / 1e3)
+    achieved_tflops = flops_per_point(L) * points / (k_avg / 1e3) / 1e12
+    img_fused = image.clone()
+    log(f"fused: {ms_per_step:.2f} ms/step, kernel {k_avg:.2f} ms, {value:.3e} points/s, "
+        f"{achieved_tflops:.1f} TFLOP/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (complex64)",
+        "data": "synthetic: seeded 512x1024 gradient+anomaly model, Ricker source, seeded receiver spectra",
+        "config": {"workload": "c2_512x1024_nf256_16shots_L31_nk2048", "nz": nz, "nx": nx,
+                   "nf": nf, "shots": ns, "taps": L, "nk": t.nk, "points_per_step": points,
+                   "path": "batched fused compact-table kernel (one launch, all shots)",
+                   "plan": info, "tables_conjugate": conj},
+        "roofline": {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: per-point gather-convolution)",
+                     "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+                     "traffic": read_profile_traffic("fused_kernel_c2_traffic.txt"),
+                     "kernel": "fused_kernel<31,4,S,conj>", "kernel_ms": k_avg,
+                     "flops_per_point": flops_per_point(L)},
+    }
+
+    # ---------------------------------------------------------------- plain
+    if not args.no_plain:
+        log("expanding tables (2 x %.1f GB) ..." % (nz * nf * nx * L * 8 / 1e9))
+        tapF = torch.empty(nz * nf * L * nx, dtype=torch.complex64, device=dev)
+        tapB = torch.empty_like(tapF)
+        device.expand_table_device(nz, nf, nx, t.M, slow, omega, t.kappa0, t.dkappa, t.nk, tf, tapF,
+                                   layout=LAYOUT_TAPMAJOR, stream=stream)
+        device.expand_table_device(nz, nf, nx, t.M, slow, omega, t.kappa0, t.dkappa, t.nk, tb, tapB,
+                                   layout=LAYOUT_TAPMAJOR, stream=stream)
+        Fp = F0.clone()
+        Bp = B0.clone()
+        pimg = torch.zeros((ns, nz, nx), dtype=torch.float32, device=dev)
+
+        def plain_step():
+            Fp.copy_(F0)
+            Bp.copy_(B0)
+            device.plain_device(ns, nz, nz, nf, nf, nx, t.M, tapF, Fp, tapB, Bp, pimg,
+                                layout=LAYOUT_TAPMAJOR, shots_per_launch=1, stream=stream)
+
+        plain_step()  # warm-up (and scratch allocation)
+        torch.cuda.synchronize()
+        pe = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.plain_steps)]
+        for e0, e1 in pe:
+            Fp.copy_(F0)
+            Bp.copy_(B0)
+            e0.record(stream)
+            device.plain_device(ns, nz, nz, nf, nf, nx, t.M, tapF, Fp, tapB, Bp, pimg,
+                                layout=LAYOUT_TAPMAJOR, shots_per_launch=1, stream=stream)
+            e1.record(stream)
+        torch.cuda.synchronize()
+        p_ms = sum(e0.elapsed_time(e1) for e0, e1 in pe) / len(pe)
+        p_value = points / (p_ms / 1e3)
+        per_launch_ms = p_ms / (ns * nz)
+        gbs = plain_bytes_per_point(L) * (nf * nx) / (per_launch_ms / 1e3) / 1e9
+        stacked = pimg.sum(0)
+        err = float(torch.linalg.norm(stacked - img_fused) / torch.linalg.norm(stacked))
+        log(f"plain: {p_ms:.1f} ms/step, {p_value:.3e} points/s, {gbs:.0f} GB/s per step launch, "
+            f"fused-vs-plain image rel L2 {err:.2e}")
+        out["plain"] = {"value": p_value, "unit": "points/s", "ms_per_step": p_ms,
+                        "path": "drop-in plain path, expanded tap-major tables in HBM, one launch "
+                                "sequence per shot", "launches_per_step": ns * (nz + 3)}
+        out["speedup_vs_plain"] = value / p_value
+        out["image_rel_l2_batched_vs_plain"] = err
+        out["roofline_plain"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                                 "traffic": read_profile_traffic("plain_step_c2_traffic.txt"),
+                                 "kernel": "plain_step_kernel<1,31>",
+                                 "bytes_per_point": plain_bytes_per_point(L),
+                                 "note": "per-launch time = step time / (shots*depths); includes "
+                                         "the small copy/reduce launches"}
+        del tapF, tapB
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(p)
+        log(out["cpu_baseline"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

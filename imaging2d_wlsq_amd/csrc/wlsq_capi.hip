@@ -1,0 +1,887 @@
+// C ABI of libwlsq_hip.so: argument checks, device scratch, the work-list
+// planner of the fused kernel, plans and graphs. Declarations and the
+// reference interfaces they replace are in include/wlsq_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "wlsq_internal.h"
+
+using namespace wlsq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? WLSQ_ENOMEM : WLSQ_EHIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+#define WLSQ_TRY(expr)      \
+    do {                    \
+        int r_ = (expr);    \
+        if (r_ != WLSQ_OK)  \
+            return r_;      \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t need)
+    {
+        if (need <= bytes)
+            return WLSQ_OK;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess)
+                return fail(WLSQ_EHIP, "hipFree: %s", hipGetErrorString(e));
+        }
+        hipError_t e = hipMalloc(&p, need);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(WLSQ_ENOMEM, "hipMalloc(%zu bytes): %s", need, hipGetErrorString(e));
+        }
+        bytes = need;
+        return WLSQ_OK;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Scratch of one stream: calls queued on one stream run in order, so they can
+// share it; calls on different streams get different scratch.
+struct Scratch {
+    DevBuf tapF, tapB, pingF[2], pingB[2], part, stage, fields_f, fields_b, img, slow, omega, tf,
+        tb;
+};
+
+struct PlanImpl;
+
+struct Device {
+    int ordinal = 0;
+    hipDeviceProp_t props{};
+    hipStream_t own_stream = nullptr;  // default priority, used by the host entries
+    std::mutex mu;           // scratch map, plan cache, device-pointer entries
+    std::mutex host_call_mu; // one host-array entry call at a time (shared scratch)
+    std::map<hipStream_t, std::unique_ptr<Scratch>> scratch;
+    std::map<std::vector<long long>, std::shared_ptr<PlanImpl>> ktable_plans;
+};
+
+std::mutex g_devices_mu;
+std::map<int, std::unique_ptr<Device>> g_devices;
+
+int current_device(Device **out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_devices_mu);
+    auto it = g_devices.find(dev);
+    if (it == g_devices.end()) {
+        auto d = std::make_unique<Device>();
+        d->ordinal = dev;
+        HIP_TRY(hipGetDeviceProperties(&d->props, dev));
+        HIP_TRY(hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
+        it = g_devices.emplace(dev, std::move(d)).first;
+    }
+    *out = it->second.get();
+    return WLSQ_OK;
+}
+
+Scratch &scratch_for(Device *d, hipStream_t st)
+{
+    auto &p = d->scratch[st];
+    if (!p)
+        p = std::make_unique<Scratch>();
+    return *p;
+}
+
+int check_common(int ns, int nextrap, int nz, int nt, int nf, int nx, int M)
+{
+    if (ns < 0 || nextrap < 0 || nz < 0 || nt < 0 || nf < 0 || nx < 0 || M < 0)
+        return fail(WLSQ_EINVAL, "negative size (ns=%d nextrap=%d nz=%d nt=%d nf=%d nx=%d M=%d)",
+                    ns, nextrap, nz, nt, nf, nx, M);
+    if (nz < nextrap)
+        return fail(WLSQ_EINVAL, "nz (%d) must be >= nextrap (%d)", nz, nextrap);
+    if (nt < nf)
+        return fail(WLSQ_EINVAL, "nt (%d) must be >= nf (%d)", nt, nf);
+    if (M > 1024)
+        return fail(WLSQ_EUNSUPPORTED, "operator half-length M=%d above 1024", M);
+    return WLSQ_OK;
+}
+
+// ------------------------------------------------------------------- plain path
+// Runs nextrap steps for `nsh` shots with S shots per thread. src/dst fields
+// have shot stride src_stride/dst_stride and row stride nx; img has shot
+// stride img_stride and row stride nx.
+int plain_sequence(Scratch &sc, int S, int nsh, int nextrap, int nf, int nx, int M,
+                   const float2 *tapF, const float2 *tapB, const float2 *srcF,
+                   const float2 *srcB, long long src_stride, float2 *dstF, float2 *dstB,
+                   long long dst_stride, float *img, long long img_stride, hipStream_t st)
+{
+    if (nextrap <= 0 || nsh <= 0)
+        return WLSQ_OK;
+    const long long plane = (long long)nf * nx;
+    const int G = plain_groups(nf);
+    const long long part_stride = (long long)nextrap * G * nx;
+    const int L = 2 * M + 1;
+    const long long depth_slice = (long long)nf * L * nx;
+    if (nf > 0) {
+        for (int b = 0; b < 2; ++b) {
+            WLSQ_TRY(sc.pingF[b].reserve(sizeof(float2) * (size_t)(nsh * plane)));
+            WLSQ_TRY(sc.pingB[b].reserve(sizeof(float2) * (size_t)(nsh * plane)));
+        }
+        WLSQ_TRY(sc.part.reserve(sizeof(float) * (size_t)(nsh * part_stride)));
+        for (int l = 0; l < nextrap; ++l) {
+            const float2 *inF = l == 0 ? srcF : sc.pingF[(l - 1) & 1].as<float2>();
+            const float2 *inB = l == 0 ? srcB : sc.pingB[(l - 1) & 1].as<float2>();
+            const long long in_stride = l == 0 ? src_stride : plane;
+            HIP_TRY(launch_plain_step(S, tapF + l * depth_slice, tapB + l * depth_slice, inF, inB,
+                                      in_stride, sc.pingF[l & 1].as<float2>(),
+                                      sc.pingB[l & 1].as<float2>(),
+                                      sc.part.as<float>() + (long long)l * G * nx, part_stride, nsh,
+                                      nf, nx, M, st));
+        }
+        const int last = (nextrap - 1) & 1;
+        HIP_TRY(launch_copy_rows(sc.pingF[last].as<float2>(), plane, dstF, dst_stride, nsh, nf, nx, st));
+        HIP_TRY(launch_copy_rows(sc.pingB[last].as<float2>(), plane, dstB, dst_stride, nsh, nf, nx, st));
+        HIP_TRY(launch_plain_reduce(sc.part.as<float>(), img, nsh, img_stride, nextrap, G, nx, st));
+    } else {
+        // no frequencies: the image rows are sums over nothing
+        for (int s = 0; s < nsh; ++s)
+            HIP_TRY(launch_fill_f32(img + s * img_stride, (long long)nextrap * nx, 0.f, st));
+    }
+    return WLSQ_OK;
+}
+
+// shots_per_launch == 1: one launch sequence per shot (the reference's
+// structure); otherwise all shots in each depth launch, S per thread.
+int plain_run(Scratch &sc, int shots_per_launch, int ns, int nextrap, int nf, int nx, int M,
+              const float2 *tapF, const float2 *tapB, const float2 *srcF, const float2 *srcB,
+              long long src_stride, float2 *dstF, float2 *dstB, long long dst_stride, float *img,
+              long long img_stride, hipStream_t st)
+{
+    if (shots_per_launch == 1) {
+        for (int s = 0; s < ns; ++s)
+            WLSQ_TRY(plain_sequence(sc, 1, 1, nextrap, nf, nx, M, tapF, tapB, srcF + s * src_stride,
+                                    srcB + s * src_stride, src_stride, dstF + s * dst_stride,
+                                    dstB + s * dst_stride, dst_stride, img + s * img_stride,
+                                    img_stride, st));
+        return WLSQ_OK;
+    }
+    return plain_sequence(sc, shots_per_launch, ns, nextrap, nf, nx, M, tapF, tapB, srcF, srcB,
+                          src_stride, dstF, dstB, dst_stride, img, img_stride, st);
+}
+
+int transpose_tables(Scratch &sc, int nextrap, int nf, int nx, int M, const float2 *natF,
+                     const float2 *natB, hipStream_t st)
+{
+    const int L = 2 * M + 1;
+    const long long slices = (long long)nextrap * nf;
+    const size_t bytes = sizeof(float2) * (size_t)slices * L * nx;
+    WLSQ_TRY(sc.tapF.reserve(bytes));
+    WLSQ_TRY(sc.tapB.reserve(bytes));
+    // launch in runs of at most 65535 slices (grid y)
+    for (long long s0 = 0; s0 < slices; s0 += 65535) {
+        const long long n = std::min<long long>(65535, slices - s0);
+        const long long off = s0 * L * nx;
+        HIP_TRY(launch_transpose_tapmajor(natF + off, sc.tapF.as<float2>() + off, n, nx, L, st));
+        HIP_TRY(launch_transpose_tapmajor(natB + off, sc.tapB.as<float2>() + off, n, nx, L, st));
+    }
+    return WLSQ_OK;
+}
+
+// ------------------------------------------------------------------- planner
+struct PlanImpl {
+    int M = 0;
+    int L = 1;
+    FusedShape shape{4, 1, 256, false};
+    size_t lds = 0;
+    long long points = 0;
+    std::vector<Group> groups;
+    std::vector<Item> items;
+    Group *d_groups = nullptr;
+    Item *d_items = nullptr;
+    int device = 0;
+    ~PlanImpl()
+    {
+        if (d_groups)
+            (void)hipFree(d_groups);
+        if (d_items)
+            (void)hipFree(d_items);
+    }
+};
+
+bool same_shape(const wlsq_shot &a, const wlsq_shot &b)
+{
+    return a.nx == b.nx && a.nz == b.nz && a.nf == b.nf && a.omega_off == b.omega_off &&
+           a.slow_off == b.slow_off && a.slow_ld == b.slow_ld && a.ld == b.ld;
+}
+
+int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_tuning *tun,
+               std::shared_ptr<PlanImpl> &out)
+{
+    if (nshots < 0 || (nshots > 0 && !shots))
+        return fail(WLSQ_EINVAL, "shots array is NULL");
+    if (M < 0 || M > 1024)
+        return fail(WLSQ_EINVAL, "invalid M=%d", M);
+    wlsq_tuning t{};
+    if (tun)
+        t = *tun;
+    auto P = std::make_shared<PlanImpl>();
+    P->M = M;
+    P->L = 2 * M + 1;
+    P->device = d->ordinal;
+
+    int max_nx = 0;
+    for (int i = 0; i < nshots; ++i) {
+        const wlsq_shot &s = shots[i];
+        if (s.nx < 0 || s.nz < 0 || s.nf < 0 || s.omega_off < 0 || s.f_off < 0 || s.b_off < 0 ||
+            s.img_off < 0 || s.slow_off < 0)
+            return fail(WLSQ_EINVAL, "shot %d: negative size or offset", i);
+        if (s.nx > 0 && s.nf > 0 && s.nz > 0) {
+            if (s.ld < s.nx || s.img_ld < s.nx || s.slow_ld < s.nx)
+                return fail(WLSQ_EINVAL, "shot %d: row stride below aperture (ld=%d img_ld=%d slow_ld=%d nx=%d)",
+                            i, s.ld, s.img_ld, s.slow_ld, s.nx);
+            max_nx = std::max(max_nx, s.nx);
+        }
+        P->points += (long long)s.nx * s.nz * s.nf;
+    }
+
+    // x per thread and shots per group
+    int R = t.x_per_thread;
+    if (R == 0)
+        R = (max_nx + 3) / 4 <= 512 ? 4 : 8;
+    if (R != 4 && R != 8)
+        return fail(WLSQ_EINVAL, "x_per_thread must be 4 or 8 (got %d)", R);
+    const int max_block = R == 8 ? 1024 : 512;
+    const int max_strips = (max_nx + R - 1) / R;
+    if (max_strips > max_block)
+        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d threads at %d x per thread (max %d)",
+                    max_nx, max_strips, R, max_block);
+    int S = t.shots_per_group;
+    if (S == 0) {
+        int run = 1, best = 1;
+        for (int i = 1; i < nshots; ++i) {
+            run = same_shape(shots[i], shots[i - 1]) ? run + 1 : 1;
+            best = std::max(best, run);
+        }
+        S = R == 8 ? 1 : (best >= 4 ? 4 : best >= 2 ? 2 : 1);
+    }
+    if (S != 1 && S != 2 && S != 4)
+        return fail(WLSQ_EINVAL, "shots_per_group must be 1, 2 or 4 (got %d)", S);
+    if (R == 8 && S != 1)
+        return fail(WLSQ_EUNSUPPORTED, "8 x per thread supports 1 shot per group");
+
+    int block = t.block;
+    if (block == 0) {
+        block = ((max_strips + 63) / 64) * 64;
+        if (block < 256)
+            block = std::min(256, max_block);
+    }
+    if (block < 64 || block > max_block || block % 64 || block < max_strips)
+        return fail(WLSQ_EINVAL, "block=%d invalid for %d strips (multiple of 64, <= %d)", block,
+                    max_strips, max_block);
+
+    const size_t lds_max = std::max((size_t)d->props.sharedMemPerBlock,
+                                    (size_t)d->props.sharedMemPerBlockOptin);
+    size_t budget = t.lds_budget > 0 ? (size_t)t.lds_budget : std::min<size_t>(lds_max, 80 * 1024);
+    if (budget > lds_max)
+        budget = lds_max;
+
+    // groups: runs of up to S same-shaped consecutive shots
+    std::vector<std::pair<int, int>> runs;  // (first shot, count)
+    for (int i = 0; i < nshots;) {
+        const wlsq_shot &s = shots[i];
+        if (s.nx == 0 || s.nf == 0 || s.nz == 0) {
+            ++i;
+            continue;
+        }
+        int n = 1;
+        while (n < S && i + n < nshots && same_shape(shots[i + n], s))
+            ++n;
+        runs.emplace_back(i, n);
+        i += n;
+    }
+    struct Pending {
+        Item item;
+        long long cost;
+    };
+    std::vector<Pending> pend;
+    size_t lds_need = 0;
+    for (auto &rn : runs) {
+        const wlsq_shot &s = shots[rn.first];
+        Group g{};
+        for (int k = 0; k < kMaxShotsPerGroup; ++k) {
+            const wlsq_shot &sk = shots[rn.first + std::min(k, rn.second - 1)];
+            g.f_off[k] = sk.f_off;
+            g.b_off[k] = sk.b_off;
+            g.img_off[k] = sk.img_off;
+        }
+        g.slow_off = s.slow_off;
+        g.nshots = rn.second;
+        g.ld = s.ld;
+        g.img_ld = s.img_ld;
+        g.slow_ld = s.slow_ld;
+        g.nx = s.nx;
+        g.nz = s.nz;
+        g.nf = s.nf;
+        g.omega_off = s.omega_off;
+        const int strips = (s.nx + R - 1) / R;
+        int W = block / strips;
+        if (t.omegas_per_item > 0)
+            W = std::min(W, t.omegas_per_item);
+        W = std::min(W, s.nf);
+        while (W > 1 && fused_lds_bytes(s.nx, M, R, S, W) > budget)
+            --W;
+        if (W < 1)
+            W = 1;
+        const size_t need = fused_lds_bytes(s.nx, M, R, S, W);
+        if (need > lds_max)
+            return fail(WLSQ_EUNSUPPORTED,
+                        "aperture nx=%d with M=%d needs %zu bytes of LDS (device max %zu)", s.nx,
+                        M, need, lds_max);
+        lds_need = std::max(lds_need, need);
+        const int gi = (int)P->groups.size();
+        P->groups.push_back(g);
+        for (int j0 = 0; j0 < s.nf; j0 += W) {
+            Pending pd;
+            pd.item.group = gi;
+            pd.item.j0 = j0;
+            pd.item.nw = std::min(W, s.nf - j0);
+            pd.item.cost_rank = 0;
+            pd.cost = (long long)s.nz * s.nx * pd.item.nw * rn.second;
+            pend.push_back(pd);
+        }
+    }
+    // longest first: the hardware dispatches workgroups in index order
+    std::stable_sort(pend.begin(), pend.end(),
+                     [](const Pending &a, const Pending &b) { return a.cost > b.cost; });
+    for (size_t i = 0; i < pend.size(); ++i) {
+        pend[i].item.cost_rank = (int)i;
+        P->items.push_back(pend[i].item);
+    }
+    P->shape = FusedShape{R, S, block, t.tables_conjugate != 0};
+    P->lds = lds_need;
+    if (!P->groups.empty()) {
+        HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
+        HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
+                          hipMemcpyHostToDevice));
+    }
+    if (!P->items.empty()) {
+        HIP_TRY(hipMalloc(&P->d_items, sizeof(Item) * P->items.size()));
+        HIP_TRY(hipMemcpy(P->d_items, P->items.data(), sizeof(Item) * P->items.size(),
+                          hipMemcpyHostToDevice));
+    }
+    out = P;
+    return WLSQ_OK;
+}
+
+int launch_plan(const PlanImpl &P, const float *slowness, const float *omega, float kappa0,
+                float dkappa, int nk, const wlsq_c32 *tf, const wlsq_c32 *tb, wlsq_c32 *forw,
+                wlsq_c32 *back, float *image, int write_back, hipStream_t st)
+{
+    if (P.items.empty())
+        return WLSQ_OK;
+    if (!slowness || !omega || !tf || (!tb && !P.shape.conj) || !forw || !back || !image)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    if (nk <= 0 || !(dkappa > 0.0f))
+        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    FusedArgs a{};
+    a.items = P.d_items;
+    a.groups = P.d_groups;
+    a.F = reinterpret_cast<float2 *>(forw);
+    a.B = reinterpret_cast<float2 *>(back);
+    a.img = image;
+    a.omega = omega;
+    a.slow = slowness;
+    a.TF = reinterpret_cast<const float2 *>(tf);
+    a.TB = reinterpret_cast<const float2 *>(tb ? tb : tf);
+    a.nk = nk;
+    a.kappa0 = kappa0;
+    a.inv_dk = inv_dkappa(dkappa);
+    a.L = P.L;
+    a.write_back = write_back;
+    HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+    return WLSQ_OK;
+}
+
+std::vector<wlsq_shot> uniform_shots(int ns, int nextrap, int nz, int nt, int nf, int nx,
+                                     int slow_ld)
+{
+    std::vector<wlsq_shot> v((size_t)ns);
+    for (int s = 0; s < ns; ++s) {
+        wlsq_shot &d = v[(size_t)s];
+        d = wlsq_shot{};
+        d.f_off = (long long)s * nt * nx;
+        d.b_off = d.f_off;
+        d.img_off = (long long)s * nz * nx;
+        d.slow_off = 0;
+        d.ld = nx;
+        d.img_ld = nx;
+        d.slow_ld = slow_ld;
+        d.nx = nx;
+        d.nz = nextrap;
+        d.nf = nf;
+        d.omega_off = 0;
+    }
+    return v;
+}
+
+int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                       const float *slowness, int slow_ld, const float *omega, float kappa0,
+                       float dkappa, int nk, const wlsq_c32 *tf, wlsq_c32 *forw,
+                       const wlsq_c32 *tb, wlsq_c32 *back, float *image, int write_back,
+                       const wlsq_tuning *tuning, hipStream_t st)
+{
+    WLSQ_TRY(check_common(ns, nextrap, nz, nt, nf, nx, M));
+    if (ns == 0 || nextrap == 0 || nx == 0)
+        return WLSQ_OK;
+    if (slow_ld < nx)
+        return fail(WLSQ_EINVAL, "slow_ld (%d) < nx (%d)", slow_ld, nx);
+    // image rows l < nextrap are outputs: clear them, then accumulate
+    for (int s = 0; s < ns; ++s)
+        HIP_TRY(launch_fill_f32(image + (long long)s * nz * nx, (long long)nextrap * nx, 0.f, st));
+    if (nf == 0)
+        return WLSQ_OK;
+    wlsq_tuning t{};
+    if (tuning)
+        t = *tuning;
+    std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
+                               t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
+                               t.tables_conjugate};
+    std::shared_ptr<PlanImpl> P;
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        auto it = d->ktable_plans.find(key);
+        if (it != d->ktable_plans.end())
+            P = it->second;
+    }
+    if (!P) {
+        auto shots = uniform_shots(ns, nextrap, nz, nt, nf, nx, slow_ld);
+        WLSQ_TRY(build_plan(d, shots.data(), ns, M, &t, P));
+        std::lock_guard<std::mutex> lk(d->mu);
+        if (d->ktable_plans.size() > 64)
+            d->ktable_plans.clear();
+        d->ktable_plans[key] = P;
+    }
+    return launch_plan(*P, slowness, omega, kappa0, dkappa, nk, tf, tb, forw, back, image,
+                       write_back, st);
+}
+
+bool host_tables_conjugate(const wlsq_c32 *tf, const wlsq_c32 *tb, long long n)
+{
+    for (long long i = 0; i < n; ++i) {
+        uint32_t a, b, c, e;
+        std::memcpy(&a, &tf[i].re, 4);
+        std::memcpy(&b, &tb[i].re, 4);
+        const float ni = -tf[i].im;
+        std::memcpy(&c, &ni, 4);
+        std::memcpy(&e, &tb[i].im, 4);
+        if (a != b || c != e)
+            return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+struct wlsq_plan_s {
+    std::shared_ptr<PlanImpl> impl;
+};
+
+struct wlsq_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::shared_ptr<PlanImpl> plan;
+};
+
+extern "C" {
+
+const char *wlsq_last_error(void) { return g_err.c_str(); }
+
+int wlsq_version(void) { return 1; }
+
+int wlsq_set_device(int device)
+{
+    HIP_TRY(hipSetDevice(device));
+    return WLSQ_OK;
+}
+
+int extrapolate(int ns, int nextrap, int nz, int nt, int nf, int nx, int M, wlsq_c32 *w_op_forw,
+                wlsq_c32 *forw_pulse, wlsq_c32 *w_op_back, wlsq_c32 *back_pulse, float *image)
+{
+    WLSQ_TRY(check_common(ns, nextrap, nz, nt, nf, nx, M));
+    if (ns == 0 || nextrap == 0 || nx == 0)
+        return WLSQ_OK;
+    if (!w_op_forw || !w_op_back || !forw_pulse || !back_pulse || !image)
+        return fail(WLSQ_EINVAL, "NULL array");
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    std::lock_guard<std::mutex> call_lk(d->host_call_mu);
+    hipStream_t st = d->own_stream;
+    Scratch *scp;
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        scp = &scratch_for(d, st);
+    }
+    Scratch &sc = *scp;
+    const int L = 2 * M + 1;
+    const long long slice = (long long)nf * nx * L;  // one depth of one table
+    const long long plane = (long long)nf * nx;
+
+    if (nf > 0) {
+        // tables: natural layout uploaded in chunks of depths, transposed to
+        // tap-major on the device (replaces cpp_src/revOp.c:4-30 on the host)
+        const size_t tab_bytes = sizeof(float2) * (size_t)(nextrap * slice);
+        WLSQ_TRY(sc.tapF.reserve(tab_bytes));
+        WLSQ_TRY(sc.tapB.reserve(tab_bytes));
+        const long long chunk_depths =
+            std::max<long long>(1, (64ll << 20) / (long long)(sizeof(float2) * slice));
+        const long long cd = std::min<long long>(chunk_depths, nextrap);
+        WLSQ_TRY(sc.stage.reserve(sizeof(float2) * (size_t)(cd * slice)));
+        for (int pass = 0; pass < 2; ++pass) {
+            const wlsq_c32 *src = pass == 0 ? w_op_forw : w_op_back;
+            float2 *dst = pass == 0 ? sc.tapF.as<float2>() : sc.tapB.as<float2>();
+            for (long long l0 = 0; l0 < nextrap; l0 += cd) {
+                const long long n = std::min<long long>(cd, nextrap - l0);
+                HIP_TRY(hipMemcpyAsync(sc.stage.p, src + l0 * slice, sizeof(float2) * (size_t)(n * slice),
+                                       hipMemcpyHostToDevice, st));
+                for (long long s0 = 0; s0 < n * nf; s0 += 65534) {
+                    const long long m = std::min<long long>(65534, n * nf - s0);
+                    HIP_TRY(launch_transpose_tapmajor(sc.stage.as<float2>() + s0 * nx * L,
+                                                      dst + l0 * slice + s0 * nx * L, m, nx, L, st));
+                }
+            }
+        }
+        // fields: rows j < nf of each shot
+        WLSQ_TRY(sc.fields_f.reserve(sizeof(float2) * (size_t)(ns * plane)));
+        WLSQ_TRY(sc.fields_b.reserve(sizeof(float2) * (size_t)(ns * plane)));
+        const size_t row_bytes = sizeof(float2) * (size_t)plane;
+        const size_t shot_pitch = sizeof(float2) * (size_t)nt * nx;
+        HIP_TRY(hipMemcpy2DAsync(sc.fields_f.p, row_bytes, forw_pulse, shot_pitch, row_bytes, ns,
+                                 hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpy2DAsync(sc.fields_b.p, row_bytes, back_pulse, shot_pitch, row_bytes, ns,
+                                 hipMemcpyHostToDevice, st));
+    }
+    WLSQ_TRY(sc.img.reserve(sizeof(float) * (size_t)ns * nextrap * nx));
+    const int spl = ns >= 4 ? 4 : ns >= 2 ? 2 : 1;
+    WLSQ_TRY(plain_run(sc, spl, ns, nextrap, nf, nx, M, sc.tapF.as<float2>(), sc.tapB.as<float2>(),
+                       sc.fields_f.as<float2>(), sc.fields_b.as<float2>(), plane,
+                       sc.fields_f.as<float2>(), sc.fields_b.as<float2>(), plane,
+                       sc.img.as<float>(), (long long)nextrap * nx, st));
+    if (nf > 0) {
+        const size_t row_bytes = sizeof(float2) * (size_t)plane;
+        const size_t shot_pitch = sizeof(float2) * (size_t)nt * nx;
+        HIP_TRY(hipMemcpy2DAsync(forw_pulse, shot_pitch, sc.fields_f.p, row_bytes, row_bytes, ns,
+                                 hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpy2DAsync(back_pulse, shot_pitch, sc.fields_b.p, row_bytes, row_bytes, ns,
+                                 hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipMemcpy2DAsync(image, sizeof(float) * (size_t)nz * nx, sc.img.p,
+                             sizeof(float) * (size_t)nextrap * nx, sizeof(float) * (size_t)nextrap * nx,
+                             ns, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return WLSQ_OK;
+}
+
+int wlsq_plain_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                      const wlsq_c32 *op_forw, wlsq_c32 *forw_pulse, const wlsq_c32 *op_back,
+                      wlsq_c32 *back_pulse, float *image, int layout, int shots_per_launch,
+                      void *stream)
+{
+    WLSQ_TRY(check_common(ns, nextrap, nz, nt, nf, nx, M));
+    if (layout != WLSQ_LAYOUT_NATURAL && layout != WLSQ_LAYOUT_TAPMAJOR)
+        return fail(WLSQ_EINVAL, "unknown table layout %d", layout);
+    if (shots_per_launch != 1 && shots_per_launch != 2 && shots_per_launch != 4)
+        return fail(WLSQ_EINVAL, "shots_per_launch must be 1, 2 or 4 (got %d)", shots_per_launch);
+    if (ns == 0 || nextrap == 0 || nx == 0)
+        return WLSQ_OK;
+    if (!op_forw || !op_back || !forw_pulse || !back_pulse || !image)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(d->mu);
+    Scratch &sc = scratch_for(d, st);
+    const float2 *tapF = reinterpret_cast<const float2 *>(op_forw);
+    const float2 *tapB = reinterpret_cast<const float2 *>(op_back);
+    if (layout == WLSQ_LAYOUT_NATURAL && nf > 0) {
+        WLSQ_TRY(transpose_tables(sc, nextrap, nf, nx, M, tapF, tapB, st));
+        tapF = sc.tapF.as<float2>();
+        tapB = sc.tapB.as<float2>();
+    }
+    const long long stride = (long long)nt * nx;
+    return plain_run(sc, shots_per_launch, ns, nextrap, nf, nx, M, tapF, tapB,
+                     reinterpret_cast<const float2 *>(forw_pulse),
+                     reinterpret_cast<const float2 *>(back_pulse), stride,
+                     reinterpret_cast<float2 *>(forw_pulse), reinterpret_cast<float2 *>(back_pulse),
+                     stride, image, (long long)nz * nx, st);
+}
+
+int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                       const float *slowness, int slow_ld, const float *omega, float kappa0,
+                       float dkappa, int nk, const wlsq_c32 *table_forw, wlsq_c32 *forw_pulse,
+                       const wlsq_c32 *table_back, wlsq_c32 *back_pulse, float *image,
+                       int write_back, const wlsq_tuning *tuning, void *stream)
+{
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    return ktable_device_impl(d, ns, nextrap, nz, nt, nf, nx, M, slowness, slow_ld, omega, kappa0,
+                              dkappa, nk, table_forw, forw_pulse, table_back, back_pulse, image,
+                              write_back, tuning, static_cast<hipStream_t>(stream));
+}
+
+int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                       const float *slowness, const float *omega, float kappa0, float dkappa,
+                       int nk, const wlsq_c32 *table_forw, wlsq_c32 *forw_pulse,
+                       const wlsq_c32 *table_back, wlsq_c32 *back_pulse, float *image)
+{
+    WLSQ_TRY(check_common(ns, nextrap, nz, nt, nf, nx, M));
+    if (ns == 0 || nextrap == 0 || nx == 0)
+        return WLSQ_OK;
+    if (!slowness || !omega || !table_forw || !table_back || !forw_pulse || !back_pulse || !image)
+        return fail(WLSQ_EINVAL, "NULL array");
+    if (nk <= 0 || !(dkappa > 0.0f))
+        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    hipStream_t st = d->own_stream;
+    const int L = 2 * M + 1;
+    const long long plane = (long long)nf * nx;
+    std::lock_guard<std::mutex> call_lk(d->host_call_mu);
+    Scratch *scp;
+    {
+        std::lock_guard<std::mutex> lk(d->mu);
+        scp = &scratch_for(d, st);
+    }
+    Scratch &sc = *scp;
+    WLSQ_TRY(sc.slow.reserve(sizeof(float) * (size_t)nextrap * nx));
+    WLSQ_TRY(sc.omega.reserve(sizeof(float) * (size_t)std::max(nf, 1)));
+    WLSQ_TRY(sc.tf.reserve(sizeof(float2) * (size_t)nk * L));
+    WLSQ_TRY(sc.tb.reserve(sizeof(float2) * (size_t)nk * L));
+    WLSQ_TRY(sc.fields_f.reserve(sizeof(float2) * (size_t)std::max(1ll, ns * plane)));
+    WLSQ_TRY(sc.fields_b.reserve(sizeof(float2) * (size_t)std::max(1ll, ns * plane)));
+    WLSQ_TRY(sc.img.reserve(sizeof(float) * (size_t)ns * nextrap * nx));
+    HIP_TRY(hipMemcpyAsync(sc.slow.p, slowness, sizeof(float) * (size_t)nextrap * nx,
+                           hipMemcpyHostToDevice, st));
+    if (nf > 0)
+        HIP_TRY(hipMemcpyAsync(sc.omega.p, omega, sizeof(float) * (size_t)nf, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(sc.tf.p, table_forw, sizeof(float2) * (size_t)nk * L, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(sc.tb.p, table_back, sizeof(float2) * (size_t)nk * L, hipMemcpyHostToDevice, st));
+    const size_t row_bytes = sizeof(float2) * (size_t)plane;
+    const size_t shot_pitch = sizeof(float2) * (size_t)nt * nx;
+    if (nf > 0) {
+        HIP_TRY(hipMemcpy2DAsync(sc.fields_f.p, row_bytes, forw_pulse, shot_pitch, row_bytes, ns,
+                                 hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpy2DAsync(sc.fields_b.p, row_bytes, back_pulse, shot_pitch, row_bytes, ns,
+                                 hipMemcpyHostToDevice, st));
+    }
+    wlsq_tuning t{};
+    t.tables_conjugate = host_tables_conjugate(table_forw, table_back, (long long)nk * L) ? 1 : 0;
+    // device layout of the fields: (ns, nf, nx), i.e. "nt" = nf; image (ns, nextrap, nx)
+    WLSQ_TRY(ktable_device_impl(d, ns, nextrap, nextrap, nf, nf, nx, M, sc.slow.as<float>(), nx,
+                                sc.omega.as<float>(), kappa0, dkappa, nk,
+                                static_cast<const wlsq_c32 *>(sc.tf.p), static_cast<wlsq_c32 *>(sc.fields_f.p),
+                                static_cast<const wlsq_c32 *>(sc.tb.p), static_cast<wlsq_c32 *>(sc.fields_b.p),
+                                sc.img.as<float>(), 1, &t, st));
+    if (nf > 0) {
+        HIP_TRY(hipMemcpy2DAsync(forw_pulse, shot_pitch, sc.fields_f.p, row_bytes, row_bytes, ns,
+                                 hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpy2DAsync(back_pulse, shot_pitch, sc.fields_b.p, row_bytes, row_bytes, ns,
+                                 hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipMemcpy2DAsync(image, sizeof(float) * (size_t)nz * nx, sc.img.p,
+                             sizeof(float) * (size_t)nextrap * nx, sizeof(float) * (size_t)nextrap * nx,
+                             ns, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return WLSQ_OK;
+}
+
+int wlsq_expand_table_device(int nextrap, int nf, int nx, int M, const float *slowness, int slow_ld,
+                             const float *omega, float kappa0, float dkappa, int nk,
+                             const wlsq_c32 *table, wlsq_c32 *out, int layout, void *stream)
+{
+    if (nextrap < 0 || nf < 0 || nx < 0 || M < 0)
+        return fail(WLSQ_EINVAL, "negative size");
+    if (layout != WLSQ_LAYOUT_NATURAL && layout != WLSQ_LAYOUT_TAPMAJOR)
+        return fail(WLSQ_EINVAL, "unknown table layout %d", layout);
+    if (nextrap == 0 || nf == 0 || nx == 0)
+        return WLSQ_OK;
+    if (nf > 65535 || nextrap > 65535)
+        return fail(WLSQ_EUNSUPPORTED, "nf and nextrap must be <= 65535");
+    if (slow_ld < nx || nk <= 0 || !(dkappa > 0.0f))
+        return fail(WLSQ_EINVAL, "invalid slowness stride or kappa grid");
+    if (!slowness || !omega || !table || !out)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    HIP_TRY(launch_expand(slowness, slow_ld, omega, kappa0, inv_dkappa(dkappa), nk,
+                          reinterpret_cast<const float2 *>(table), reinterpret_cast<float2 *>(out),
+                          nextrap, nf, nx, 2 * M + 1, layout, static_cast<hipStream_t>(stream)));
+    return WLSQ_OK;
+}
+
+long long wlsq_batch_points(const wlsq_shot *shots, int nshots)
+{
+    long long p = 0;
+    for (int i = 0; i < nshots; ++i)
+        p += (long long)shots[i].nx * shots[i].nz * shots[i].nf;
+    return p;
+}
+
+int wlsq_plan_create(wlsq_plan *out, const wlsq_shot *shots, int nshots, int M,
+                     const wlsq_tuning *tuning)
+{
+    if (!out)
+        return fail(WLSQ_EINVAL, "NULL plan pointer");
+    *out = nullptr;
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    std::shared_ptr<PlanImpl> P;
+    WLSQ_TRY(build_plan(d, shots, nshots, M, tuning, P));
+    *out = new wlsq_plan_s{P};
+    return WLSQ_OK;
+}
+
+int wlsq_plan_info(wlsq_plan plan, long long *points, int *n_items, int *block, int *lds_bytes,
+                   int *x_per_thread, int *shots_per_group)
+{
+    if (!plan || !plan->impl)
+        return fail(WLSQ_EINVAL, "NULL plan");
+    const PlanImpl &P = *plan->impl;
+    if (points)
+        *points = P.points;
+    if (n_items)
+        *n_items = (int)P.items.size();
+    if (block)
+        *block = P.shape.block;
+    if (lds_bytes)
+        *lds_bytes = (int)P.lds;
+    if (x_per_thread)
+        *x_per_thread = P.shape.R;
+    if (shots_per_group)
+        *shots_per_group = P.shape.S;
+    return WLSQ_OK;
+}
+
+int wlsq_plan_launch(wlsq_plan plan, const float *slowness, const float *omega, float kappa0,
+                     float dkappa, int nk, const wlsq_c32 *table_forw, const wlsq_c32 *table_back,
+                     wlsq_c32 *forw, wlsq_c32 *back, float *image, int write_back, void *stream)
+{
+    if (!plan || !plan->impl)
+        return fail(WLSQ_EINVAL, "NULL plan");
+    return launch_plan(*plan->impl, slowness, omega, kappa0, dkappa, nk, table_forw, table_back,
+                       forw, back, image, write_back, static_cast<hipStream_t>(stream));
+}
+
+int wlsq_plan_destroy(wlsq_plan plan)
+{
+    delete plan;
+    return WLSQ_OK;
+}
+
+int wlsq_batch_device(const wlsq_shot *shots, int nshots, int M, const float *slowness,
+                      const float *omega, float kappa0, float dkappa, int nk,
+                      const wlsq_c32 *table_forw, const wlsq_c32 *table_back, wlsq_c32 *forw,
+                      wlsq_c32 *back, float *image, int write_back, const wlsq_tuning *tuning,
+                      void *stream)
+{
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    std::shared_ptr<PlanImpl> P;
+    WLSQ_TRY(build_plan(d, shots, nshots, M, tuning, P));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    WLSQ_TRY(launch_plan(*P, slowness, omega, kappa0, dkappa, nk, table_forw, table_back, forw,
+                         back, image, write_back, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return WLSQ_OK;
+}
+
+int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, const float *omega,
+                      float kappa0, float dkappa, int nk, const wlsq_c32 *table_forw,
+                      const wlsq_c32 *table_back, wlsq_c32 *forw, wlsq_c32 *back, float *image,
+                      int write_back, long long zero_count)
+{
+    if (!out || !plan || !plan->impl)
+        return fail(WLSQ_EINVAL, "NULL plan or output");
+    *out = nullptr;
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    hipStream_t cap = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    auto g = std::make_unique<wlsq_graph_s>();
+    g->plan = plan->impl;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    int rc = WLSQ_OK;
+    if (e == hipSuccess) {
+        if (zero_count > 0) {
+            hipError_t fe = launch_fill_f32(image, zero_count, 0.f, cap);
+            if (fe != hipSuccess)
+                rc = fail(WLSQ_EHIP, "fill in capture: %s", hipGetErrorString(fe));
+        }
+        if (rc == WLSQ_OK)
+            rc = launch_plan(*plan->impl, slowness, omega, kappa0, dkappa, nk, table_forw,
+                             table_back, forw, back, image, write_back, cap);
+        hipError_t ee = hipStreamEndCapture(cap, &g->graph);
+        if (rc == WLSQ_OK && ee != hipSuccess)
+            rc = fail(WLSQ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    } else {
+        rc = fail(WLSQ_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    (void)hipStreamDestroy(cap);
+    if (rc != WLSQ_OK) {
+        if (g->graph)
+            (void)hipGraphDestroy(g->graph);
+        return rc;
+    }
+    HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return WLSQ_OK;
+}
+
+int wlsq_graph_launch(wlsq_graph graph, void *stream)
+{
+    if (!graph || !graph->exec)
+        return fail(WLSQ_EINVAL, "NULL graph");
+    HIP_TRY(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)));
+    return WLSQ_OK;
+}
+
+int wlsq_graph_destroy(wlsq_graph graph)
+{
+    if (!graph)
+        return WLSQ_OK;
+    if (graph->exec)
+        (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph)
+        (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return WLSQ_OK;
+}
+
+}  // extern "C"

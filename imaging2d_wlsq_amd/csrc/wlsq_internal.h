@@ -1,0 +1,96 @@
+// Internal declarations shared by the kernels and the C ABI of libwlsq_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "wlsq_hip.h"
+
+namespace wlsq {
+
+constexpr int kMaxShotsPerGroup = 4;
+
+// One shot group: up to 4 shots that share aperture, depth count and
+// frequency list, so one coefficient load serves all of them.
+struct Group {
+    long long f_off[kMaxShotsPerGroup];   // row 0 of each shot's forward field
+    long long b_off[kMaxShotsPerGroup];   // row 0 of each shot's backward field
+    long long img_off[kMaxShotsPerGroup]; // image row 0 of each shot
+    long long slow_off;                   // slowness (depth 0, window column 0)
+    int nshots;                           // 1..S
+    int ld;                               // field row stride
+    int img_ld;                           // image row stride
+    int slow_ld;                          // slowness row stride
+    int nx;                               // aperture
+    int nz;                               // depth steps
+    int nf;                               // frequencies
+    int omega_off;                        // first frequency in the omega pool
+};
+
+// One workgroup task: `nw` consecutive frequencies (from `j0`) of one group,
+// all depth steps.
+struct Item {
+    int group;
+    int j0;
+    int nw;
+    int cost_rank;
+};
+
+struct FusedArgs {
+    const Item *items;
+    const Group *groups;
+    float2 *F;
+    float2 *B;
+    float *img;
+    const float *omega;
+    const float *slow;
+    const float2 *TF;
+    const float2 *TB;
+    int nk;
+    float kappa0;
+    float inv_dk;
+    int L;
+    int write_back;
+};
+
+// Launch shape of the fused kernel; chosen by the planner.
+struct FusedShape {
+    int R;      // consecutive x per thread (4 or 8)
+    int S;      // shots per group (1, 2, 4)
+    int block;  // threads per workgroup
+    bool conj;  // table_back == conj(table_forw): load one table
+};
+
+// LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of
+// R sub-rows, plus an image row per shot when W > 1.
+size_t fused_lds_bytes(int nx, int M, int R, int S, int W);
+
+hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,
+                        const FusedArgs &a);
+
+// Plain path kernels.
+hipError_t launch_transpose_tapmajor(const float2 *nat, float2 *tap, long long n_slices, int nx,
+                                     int L, hipStream_t st);
+// Frequency groups of one plain step (blocks along omega); part holds
+// [shot][depth][group][x] floats.
+int plain_groups(int nf);
+hipError_t launch_plain_step(int S, const float2 *opF, const float2 *opB, const float2 *oldF,
+                             const float2 *oldB, long long in_shot_stride, float2 *newF,
+                             float2 *newB, float *part, long long part_stride, int nshots, int nf,
+                             int nx, int M, hipStream_t st);
+hipError_t launch_plain_reduce(const float *part, float *image, int nshots, long long image_shot_stride,
+                               int nextrap, int groups, int nx, hipStream_t st);
+hipError_t launch_copy_rows(const float2 *src, long long src_shot_stride, float2 *dst,
+                            long long dst_shot_stride, int nshots, int nrows, int nx,
+                            hipStream_t st);
+hipError_t launch_expand(const float *slow, int slow_ld, const float *omega, float kappa0,
+                         float inv_dk, int nk, const float2 *table, float2 *out, int nextrap,
+                         int nf, int nx, int L, int layout, hipStream_t st);
+hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+
+// (float)(1.0 / (double)dkappa): the reciprocal every implementation of the
+// kappa lookup uses.
+inline float inv_dkappa(float dkappa) { return (float)(1.0 / (double)dkappa); }
+
+}  // namespace wlsq

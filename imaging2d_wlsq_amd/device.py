@@ -1,0 +1,195 @@
+"""Device-pointer entries of libwlsq_hip.so for data resident in HBM.
+
+PyTorch is only the allocator and stream provider here: tensors are passed as
+raw pointers, and every computation runs in the HIP library. Complex data is
+``torch.complex64`` (8-byte interleaved, the layout of ``wlsq_c32``).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_int, c_longlong, c_void_p
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from ._lib import LAYOUT_NATURAL, LAYOUT_TAPMAJOR, Shot, Tuning, check, lib
+
+
+def _ptr(t: torch.Tensor, dtype, name: str) -> int:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.device.type != "cuda":
+        raise TypeError(f"{name} must be on the GPU")
+    if not t.is_contiguous():
+        raise TypeError(f"{name} must be contiguous")
+    return t.data_ptr()
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, torch.cuda.Stream):
+        return stream.cuda_stream
+    return int(stream)
+
+
+def _tuning(t: Optional[dict]) -> Tuning:
+    tun = Tuning()
+    for k, v in (t or {}).items():
+        setattr(tun, k, int(v))
+    return tun
+
+
+def plain_device(ns, nextrap, nz, nt, nf, nx, M, op_forw, forw, op_back, back, image,
+                 layout=LAYOUT_TAPMAJOR, shots_per_launch=1, stream=None):
+    """Plain path on device tensors: one launch per depth step streaming the
+    expanded (nextrap, nf, [L, nx] | [nx, L]) tables; fields (ns, nt, nx) in
+    place; image (ns, nz, nx) rows < nextrap written."""
+    c = torch.complex64
+    check(lib().wlsq_plain_device(ns, nextrap, nz, nt, nf, nx, M, _ptr(op_forw, c, "op_forw"),
+                                  _ptr(forw, c, "forw"), _ptr(op_back, c, "op_back"),
+                                  _ptr(back, c, "back"), _ptr(image, torch.float32, "image"),
+                                  layout, shots_per_launch, _stream(stream)))
+
+
+def ktable_device(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, dkappa, nk,
+                  table_forw, forw, table_back, back, image, write_back=True, tuning=None,
+                  stream=None, slow_ld=None):
+    """Fused compact-table path on device tensors. image rows < nextrap of
+    every shot are cleared and then accumulated."""
+    c = torch.complex64
+    tun = _tuning(tuning)
+    check(lib().wlsq_ktable_device(ns, nextrap, nz, nt, nf, nx, M,
+                                   _ptr(slowness, torch.float32, "slowness"),
+                                   int(slow_ld if slow_ld is not None else nx),
+                                   _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
+                                   int(nk), _ptr(table_forw, c, "table_forw"), _ptr(forw, c, "forw"),
+                                   _ptr(table_back, c, "table_back"), _ptr(back, c, "back"),
+                                   _ptr(image, torch.float32, "image"), int(bool(write_back)),
+                                   byref(tun), _stream(stream)))
+
+
+def expand_table_device(nextrap, nf, nx, M, slowness, omega, kappa0, dkappa, nk, table, out,
+                        layout=LAYOUT_TAPMAJOR, stream=None, slow_ld=None):
+    c = torch.complex64
+    check(lib().wlsq_expand_table_device(nextrap, nf, nx, M, _ptr(slowness, torch.float32, "slowness"),
+                                         int(slow_ld if slow_ld is not None else nx),
+                                         _ptr(omega, torch.float32, "omega"), float(kappa0),
+                                         float(dkappa), int(nk), _ptr(table, c, "table"),
+                                         _ptr(out, c, "out"), int(layout), _stream(stream)))
+
+
+def tables_conjugate(table_forw: torch.Tensor, table_back: torch.Tensor) -> bool:
+    """True when table_back == conj(table_forw) bit for bit (then the fused
+    kernel reads one table)."""
+    a = torch.view_as_real(table_forw).view(torch.int32)
+    b = torch.view_as_real(table_back).view(torch.int32)
+    neg = torch.view_as_real(table_forw.conj().resolve_conj()).view(torch.int32)
+    return bool(torch.equal(a[..., 0], b[..., 0]) and torch.equal(neg[..., 1], b[..., 1]))
+
+
+@dataclass
+class ShotSpec:
+    """One shot of a batch (element offsets into the pools)."""
+    f_off: int
+    b_off: int
+    img_off: int
+    slow_off: int
+    ld: int
+    img_ld: int
+    slow_ld: int
+    nx: int
+    nz: int
+    nf: int
+    omega_off: int
+
+
+def _shots_array(shots: Sequence[ShotSpec]):
+    arr = (Shot * max(1, len(shots)))()
+    for i, s in enumerate(shots):
+        arr[i] = Shot(s.f_off, s.b_off, s.img_off, s.slow_off, s.ld, s.img_ld, s.slow_ld, s.nx,
+                      s.nz, s.nf, s.omega_off, 0)
+    return arr
+
+
+class Plan:
+    """Work list of a batch of uneven shots for the single fused launch."""
+
+    def __init__(self, shots: Sequence[ShotSpec], M: int, tuning: Optional[dict] = None):
+        self._h = c_void_p()
+        self.shots = list(shots)
+        self.M = M
+        arr = _shots_array(self.shots)
+        self._tun = _tuning(tuning)
+        check(lib().wlsq_plan_create(byref(self._h), arr, len(self.shots), M, byref(self._tun)))
+
+    def info(self) -> dict:
+        p = c_longlong()
+        n, b, lds, r, s = c_int(), c_int(), c_int(), c_int(), c_int()
+        check(lib().wlsq_plan_info(self._h, byref(p), byref(n), byref(b), byref(lds), byref(r), byref(s)))
+        return dict(points=p.value, items=n.value, block=b.value, lds_bytes=lds.value,
+                    x_per_thread=r.value, shots_per_group=s.value)
+
+    def launch(self, slowness, omega, kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
+               write_back=False, stream=None):
+        c = torch.complex64
+        check(lib().wlsq_plan_launch(self._h, _ptr(slowness, torch.float32, "slowness"),
+                                     _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
+                                     int(nk), _ptr(table_forw, c, "table_forw"),
+                                     _ptr(table_back, c, "table_back"), _ptr(forw, c, "forw"),
+                                     _ptr(back, c, "back"), _ptr(image, torch.float32, "image"),
+                                     int(bool(write_back)), _stream(stream)))
+
+    def graph(self, slowness, omega, kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
+              write_back=False, zero_count=0) -> "Graph":
+        return Graph(self, slowness, omega, kappa0, dkappa, nk, table_forw, table_back, forw, back,
+                     image, write_back, zero_count)
+
+    def close(self):
+        if self._h:
+            lib().wlsq_plan_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """hipGraph of [image clear] + one plan launch; pointers are fixed."""
+
+    def __init__(self, plan: Plan, slowness, omega, kappa0, dkappa, nk, table_forw, table_back,
+                 forw, back, image, write_back=False, zero_count=0):
+        c = torch.complex64
+        self._h = c_void_p()
+        self._keep = (plan, slowness, omega, table_forw, table_back, forw, back, image)
+        check(lib().wlsq_graph_create(byref(self._h), plan._h,
+                                      _ptr(slowness, torch.float32, "slowness"),
+                                      _ptr(omega, torch.float32, "omega"), float(kappa0),
+                                      float(dkappa), int(nk), _ptr(table_forw, c, "table_forw"),
+                                      _ptr(table_back, c, "table_back"), _ptr(forw, c, "forw"),
+                                      _ptr(back, c, "back"), _ptr(image, torch.float32, "image"),
+                                      int(bool(write_back)), int(zero_count)))
+
+    def launch(self, stream=None):
+        check(lib().wlsq_graph_launch(self._h, _stream(stream)))
+
+    def close(self):
+        if self._h:
+            lib().wlsq_graph_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_points(shots: Sequence[ShotSpec]) -> int:
+    return int(lib().wlsq_batch_points(_shots_array(shots), len(shots)))

@@ -1,0 +1,201 @@
+/*
+ * libwlsq_hip.so - C ABI of the MI355X (gfx950) extrapolate-and-image library.
+ *
+ * Every entry point returns an int status (WLSQ_OK = 0). The reference's
+ * entries return void and ctypes binds them with restype = c_void_p
+ * (cpp_src/interface_cuda.py:31), so an int status is compatible with that
+ * binding; wlsq_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * Types: wlsq_c32 is complex64 (8 bytes, interleaved re, im), the layout of
+ * thrust::complex<float> in cpp_src/types.h:4-9 and of numpy.complex64.
+ * Sizes are int as in the reference; all offsets inside are 64-bit.
+ *
+ * "host" entries take host arrays and copy in and out (the reference's
+ * contract: caller-owned numpy buffers, cpp_src/extrapolation_revOp_gpu.cu:78-231);
+ * "_device" entries take device pointers and a hipStream_t passed as void*
+ * and return once the work is queued on that stream.
+ */
+#ifndef WLSQ_HIP_H
+#define WLSQ_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    float re, im;
+} wlsq_c32;
+
+enum {
+    WLSQ_OK = 0,
+    WLSQ_EINVAL = 1,      /* invalid argument (shape, pointer, layout) */
+    WLSQ_EHIP = 2,        /* a HIP runtime call failed */
+    WLSQ_ENOMEM = 3,      /* device allocation failed or over the memory budget */
+    WLSQ_EUNSUPPORTED = 4 /* shape outside what the kernels support (e.g. LDS) */
+};
+
+/* operator-table layouts of the reference-shaped ("expanded") tables */
+enum {
+    WLSQ_LAYOUT_NATURAL = 0, /* (nextrap, nf, nx, L): preparation.py:103, the caller's layout */
+    WLSQ_LAYOUT_TAPMAJOR = 1 /* (nextrap, nf, L, nx): cpp_src/revOp.c:4-30 ("revOp") */
+};
+
+const char *wlsq_last_error(void);
+int wlsq_version(void);
+
+/* Device ordinal used by every later call of this thread (default 0). */
+int wlsq_set_device(int device);
+
+/*
+ * Drop-in replacement of the GPU entry
+ *   void extrapolate(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+ *                    fcomp* w_op_forw, fcomp* forw_pulse, fcomp* w_op_back,
+ *                    fcomp* back_pulse, float* image)
+ * of cpp_src/extrapolation_revOp_gpu.cu:78-80 (same signature in
+ * cpp_src/extrapolation_gpu.cu:74-76 and cpp_src/extrapolation_sh_revOp_gpu.cu:87-89),
+ * bound by cpp_src/interface_cuda.py:30-38 and called from main.py:138-150.
+ *   w_op_forw/back: (nextrap, nf, nx, 2M+1) host, natural layout
+ *   forw/back_pulse: (ns, nt, nx) host; rows j < nf are read and overwritten
+ *                    with the final wavefields (shot stride nt*nx, as
+ *                    cpp_src/extrapolation_cpu.cpp:91-98)
+ *   image: (ns, nz, nx) host; rows l < nextrap are written with
+ *          Re(sum_j F conj(B)) after step l; rows >= nextrap are untouched.
+ * Synchronous, like the reference (cudaDeviceSynchronize at :180).
+ * interface_cuda.py binds the symbol `extrapolate` of three libraries under
+ * three Python names (:16-43); this one symbol serves all three.
+ */
+int extrapolate(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                wlsq_c32 *w_op_forw, wlsq_c32 *forw_pulse, wlsq_c32 *w_op_back,
+                wlsq_c32 *back_pulse, float *image);
+
+/*
+ * The plain path on device pointers ("one launch sequence per shot" when
+ * shots_per_launch == 1): per depth step one fused kernel updates both fields
+ * of shots_per_launch shots for every (omega, x), streaming the expanded
+ * tables; per-depth imaging partials are reduced in fixed order at the end.
+ * Replaces the hot loop of cpp_src/extrapolation_revOp_gpu.cu:142-179
+ * (kernels :16-72). Tables in `layout` (natural tables are transposed on the
+ * device into a library scratch buffer first, cpp_src/revOp.c:4-30).
+ * shots_per_launch: 1, 2 or 4 (shots sharing one coefficient read).
+ */
+int wlsq_plain_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                      const wlsq_c32 *op_forw, wlsq_c32 *forw_pulse,
+                      const wlsq_c32 *op_back, wlsq_c32 *back_pulse, float *image,
+                      int layout, int shots_per_launch, void *stream);
+
+/*
+ * Compact-table ("k-table") form of the same computation: the per-point
+ * operator is gathered on the device from (nk, 2M+1) tables by
+ *   q = nearest kappa sample to omega_j * slowness[l][i]
+ * (the lookup of py_src/operatorsPreparation/preparation.py:109 with
+ * util.py:5-8; kappa_q = kappa0 + q*dkappa; ties -> lower q; evaluated in
+ * FP32 as t = omega*s; t -= kappa0; u = t*inv_dk; clamp [0, nk-1];
+ * q = ceil(u - 0.5) with inv_dk = (float)(1.0/(double)dkappa)).
+ * Gives bit-identical tables to expanding with the same rule
+ * (wlsq_expand_table_device), i.e. the same images as `extrapolate` on the
+ * expanded table.
+ *   slowness: (>= nextrap rows, slow_ld) f32, row l used at step l, columns 0..nx-1
+ *   omega: (nf) f32 rad/s
+ * The fused kernel keeps each (shot group, omega group) row resident in LDS
+ * for all nextrap steps and accumulates the image with atomics.
+ */
+int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                       const float *slowness, const float *omega, float kappa0, float dkappa,
+                       int nk, const wlsq_c32 *table_forw, wlsq_c32 *forw_pulse,
+                       const wlsq_c32 *table_back, wlsq_c32 *back_pulse, float *image);
+
+typedef struct {
+    int shots_per_group; /* 0 = auto, else 1, 2 or 4 shots sharing coefficient loads */
+    int omegas_per_item; /* 0 = auto: frequencies per workgroup task */
+    int block;           /* 0 = auto: threads per workgroup (64..1024) */
+    int x_per_thread;    /* 0 = auto: 4 or 8 consecutive x per thread */
+    int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
+    int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
+                             bit for bit, so only table_forw is read */
+} wlsq_tuning;
+
+int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                       const float *slowness, int slow_ld, const float *omega,
+                       float kappa0, float dkappa, int nk,
+                       const wlsq_c32 *table_forw, wlsq_c32 *forw_pulse,
+                       const wlsq_c32 *table_back, wlsq_c32 *back_pulse, float *image,
+                       int write_back, const wlsq_tuning *tuning, void *stream);
+
+/*
+ * Expansion of a compact table into the reference-shaped table
+ * (replaces makeTableOfOperators, preparation.py:101-115, for the symmetric
+ * operator of every x, main.py:66): out[l][j][i][:] = table[q(l,j,i)][:] in
+ * `layout`. slowness row stride slow_ld.
+ */
+int wlsq_expand_table_device(int nextrap, int nf, int nx, int M, const float *slowness,
+                             int slow_ld, const float *omega, float kappa0, float dkappa,
+                             int nk, const wlsq_c32 *table, wlsq_c32 *out, int layout,
+                             void *stream);
+
+/*
+ * Batched form: many shots of uneven aperture, depth count and frequency
+ * list in ONE launch (no counterpart in the reference, whose GPU entries take
+ * one common shape for all shots, cpp_src/extrapolation_revOp_gpu.cu:78-80).
+ * Each shot reads slowness rows from slowness + slow_off (window origin,
+ * row stride slow_ld), its frequencies from omega + omega_off, its fields
+ * from forw/back + f_off/b_off (row j at + j*ld), and adds its image into
+ * image + img_off (row l at + l*img_ld): point every shot at the same image
+ * for a stacked image.
+ */
+typedef struct {
+    long long f_off, b_off; /* element offsets of row 0 of the forward/backward fields */
+    long long img_off;      /* element offset of image row 0 */
+    long long slow_off;     /* element offset of (depth 0, window column 0) in slowness */
+    int ld;                 /* field row stride (elements) */
+    int img_ld;             /* image row stride */
+    int slow_ld;            /* slowness row stride */
+    int nx;                 /* aperture (window width) */
+    int nz;                 /* depth steps */
+    int nf;                 /* frequencies */
+    int omega_off;          /* offset of the frequency list in omega */
+    int reserved;
+} wlsq_shot;
+
+/* One call: plan, launch, wait, free (synchronous). Adds into `image`. */
+int wlsq_batch_device(const wlsq_shot *shots, int nshots, int M,
+                      const float *slowness, const float *omega, float kappa0, float dkappa,
+                      int nk, const wlsq_c32 *table_forw, const wlsq_c32 *table_back,
+                      wlsq_c32 *forw, wlsq_c32 *back, float *image, int write_back,
+                      const wlsq_tuning *tuning, void *stream);
+
+/* Reusable plan: the work list (shot groups x frequency groups, longest
+ * first) and launch shape are built once and kept on the device. */
+typedef struct wlsq_plan_s *wlsq_plan;
+int wlsq_plan_create(wlsq_plan *out, const wlsq_shot *shots, int nshots, int M,
+                     const wlsq_tuning *tuning);
+/* points, workgroup tasks, threads per workgroup, LDS bytes, x per thread,
+ * shots per group; any pointer may be NULL */
+int wlsq_plan_info(wlsq_plan plan, long long *points, int *n_items, int *block, int *lds_bytes,
+                   int *x_per_thread, int *shots_per_group);
+/* Queue the plan's single launch on `stream`; adds into `image`. */
+int wlsq_plan_launch(wlsq_plan plan, const float *slowness, const float *omega, float kappa0,
+                     float dkappa, int nk, const wlsq_c32 *table_forw,
+                     const wlsq_c32 *table_back, wlsq_c32 *forw, wlsq_c32 *back, float *image,
+                     int write_back, void *stream);
+int wlsq_plan_destroy(wlsq_plan plan);
+
+/* hipGraph of [zero `zero_count` floats at `image` (fill kernel, no memset
+ * node)] + the plan's launch, captured once and replayed by
+ * wlsq_graph_launch. The pointers are baked into the graph. */
+typedef struct wlsq_graph_s *wlsq_graph;
+int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, const float *omega,
+                      float kappa0, float dkappa, int nk, const wlsq_c32 *table_forw,
+                      const wlsq_c32 *table_back, wlsq_c32 *forw, wlsq_c32 *back, float *image,
+                      int write_back, long long zero_count);
+int wlsq_graph_launch(wlsq_graph graph, void *stream);
+int wlsq_graph_destroy(wlsq_graph graph);
+
+/* Points (x, z, omega, shot) the batched form computes for these shots. */
+long long wlsq_batch_points(const wlsq_shot *shots, int nshots);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WLSQ_HIP_H */
