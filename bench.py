@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--omegas-per-item", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--x-per-thread", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
@@ -140,6 +142,13 @@ def main():
     F0 = F0.view(ns, nf, nx)
     B0 = B0.view(ns, nf, nx)
     stream = torch.cuda.current_stream()
+    if args.calib_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)  # 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        del a, b
 
     # ---------------------------------------------------------------- batched
     specs = [device.ShotSpec(f_off=s * nf * nx, b_off=s * nf * nx, img_off=0, slow_off=0, ld=nx,

@@ -223,6 +223,7 @@ struct PlanImpl {
     int L = 1;
     FusedShape shape{4, 1, 256, false};
     size_t lds = 0;
+    int img_lds = 1;
     long long points = 0;
     std::vector<Group> groups;
     std::vector<Item> items;
@@ -241,7 +242,8 @@ struct PlanImpl {
 bool same_shape(const wlsq_shot &a, const wlsq_shot &b)
 {
     return a.nx == b.nx && a.nz == b.nz && a.nf == b.nf && a.omega_off == b.omega_off &&
-           a.slow_off == b.slow_off && a.slow_ld == b.slow_ld && a.ld == b.ld;
+           a.slow_off == b.slow_off && a.slow_ld == b.slow_ld && a.ld == b.ld &&
+           a.img_ld == b.img_ld;
 }
 
 int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_tuning *tun,
@@ -333,6 +335,27 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         Item item;
         long long cost;
     };
+    auto run_shared = [&](const std::pair<int, int> &rn) {
+        for (int k = 1; k < rn.second; ++k)
+            if (shots[rn.first + k].img_off != shots[rn.first].img_off)
+                return false;
+        return true;
+    };
+    // image rows reduced in LDS unless some group cannot fit them at W = 1
+    if (t.image_mode < 0 || t.image_mode > 2)
+        return fail(WLSQ_EINVAL, "image_mode must be 0, 1 or 2 (got %d)", t.image_mode);
+    int img_lds = t.image_mode == 2 ? 0 : 1;
+    for (auto &rn : runs) {
+        if (!img_lds)
+            break;
+        const int rows = run_shared(rn) ? 1 : S;
+        if (fused_lds_bytes(shots[rn.first].nx, M, R, S, 1, rows) > lds_max) {
+            if (t.image_mode == 1)
+                return fail(WLSQ_EUNSUPPORTED, "image rows do not fit in LDS for nx=%d",
+                            shots[rn.first].nx);
+            img_lds = 0;
+        }
+    }
     std::vector<Pending> pend;
     size_t lds_need = 0;
     for (auto &rn : runs) {
@@ -353,16 +376,19 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         g.nz = s.nz;
         g.nf = s.nf;
         g.omega_off = s.omega_off;
+        g.img_shared = run_shared(rn) ? 1 : 0;
+        g.pad_ = 0;
+        const int img_rows = img_lds ? (g.img_shared ? 1 : S) : 0;
         const int strips = (s.nx + R - 1) / R;
         int W = block / strips;
         if (t.omegas_per_item > 0)
             W = std::min(W, t.omegas_per_item);
         W = std::min(W, s.nf);
-        while (W > 1 && fused_lds_bytes(s.nx, M, R, S, W) > budget)
+        while (W > 1 && fused_lds_bytes(s.nx, M, R, S, W, img_rows) > budget)
             --W;
         if (W < 1)
             W = 1;
-        const size_t need = fused_lds_bytes(s.nx, M, R, S, W);
+        const size_t need = fused_lds_bytes(s.nx, M, R, S, W, img_rows);
         if (need > lds_max)
             return fail(WLSQ_EUNSUPPORTED,
                         "aperture nx=%d with M=%d needs %zu bytes of LDS (device max %zu)", s.nx,
@@ -389,6 +415,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     }
     P->shape = FusedShape{R, S, block, t.tables_conjugate != 0};
     P->lds = lds_need;
+    P->img_lds = img_lds;
     if (!P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
@@ -428,6 +455,7 @@ int launch_plan(const PlanImpl &P, const float *slowness, const float *omega, fl
     a.inv_dk = inv_dkappa(dkappa);
     a.L = P.L;
     a.write_back = write_back;
+    a.img_lds = P.img_lds;
     HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
     return WLSQ_OK;
 }
@@ -475,7 +503,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
         t = *tuning;
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
-                               t.tables_conjugate};
+                               t.tables_conjugate, t.image_mode};
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);

@@ -26,6 +26,8 @@ struct Group {
     int nz;                               // depth steps
     int nf;                               // frequencies
     int omega_off;                        // first frequency in the omega pool
+    int img_shared;                       // all shots add into the same image rows
+    int pad_;
 };
 
 // One workgroup task: `nw` consecutive frequencies (from `j0`) of one group,
@@ -52,6 +54,7 @@ struct FusedArgs {
     float inv_dk;
     int L;
     int write_back;
+    int img_lds;  // 1: reduce image rows in LDS, flush with lane-contiguous atomics
 };
 
 // Launch shape of the fused kernel; chosen by the planner.
@@ -63,8 +66,8 @@ struct FusedShape {
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of
-// R sub-rows, plus an image row per shot when W > 1.
-size_t fused_lds_bytes(int nx, int M, int R, int S, int W);
+// R sub-rows, plus `img_rows` image rows of the padded aperture.
+size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows);
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,
                         const FusedArgs &a);

@@ -105,8 +105,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         rowsF[o] = vf;
         rowsB[o] = vb;
     }
-    if (W > 1)
-        for (int e = tid; e < S * nxr; e += NT)
+    const bool shared_img = gp->img_shared != 0;
+    const int img_rows = shared_img ? 1 : nshots;  // image rows reduced in LDS
+    if (a.img_lds)
+        for (int e = tid; e < img_rows * nxr; e += NT)
             imgrow[e] = 0.f;
 
     long long ioff[S];
@@ -202,19 +204,43 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 for (int k = Lc; k < L; ++k)
                     tap(k);
             }
-            // imaging condition of this step, summed over the task's frequencies
-#pragma unroll
-            for (int s = 0; s < S; ++s) {
-                if (s < nshots) {
+            // imaging condition of this step. With img_lds the task's rows are
+            // summed in LDS (over its frequencies, and over its shots when they
+            // share an image) and flushed once per (depth, x) below.
+            if (a.img_lds) {
+                if (shared_img) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        const int x = x0 + r;
-                        if (x < nx) {
-                            const float v = re_mul_conj(accF[s][r], accB[s][r]);
-                            if (W > 1)
-                                atomicAdd(&imgrow[s * nxr + x], v);
-                            else
-                                unsafeAtomicAdd(a.img + ioff[s] + (long long)l * img_ld + x, v);
+                        float v = 0.f;
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                            if (s < nshots)
+                                v += re_mul_conj(accF[s][r], accB[s][r]);
+                        if (x0 + r < nx)
+                            atomicAdd(&imgrow[x0 + r], v);
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        if (s < nshots) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                if (x0 + r < nx)
+                                    atomicAdd(&imgrow[s * nxr + x0 + r],
+                                              re_mul_conj(accF[s][r], accB[s][r]));
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s < nshots) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int x = x0 + r;
+                            if (x < nx)
+                                unsafeAtomicAdd(a.img + ioff[s] + (long long)l * img_ld + x,
+                                                re_mul_conj(accF[s][r], accB[s][r]));
                         }
                     }
                 }
@@ -235,8 +261,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 }
             }
         }
-        if (W > 1) {
-            for (int e = tid; e < nshots * nx; e += NT) {
+        if (a.img_lds) {
+            // consecutive lanes flush consecutive x: one 32-byte sector of the
+            // image carries 8 atomic adds
+            for (int e = tid; e < img_rows * nx; e += NT) {
                 const int s = e / nx;
                 const int x = e - s * nx;
                 const float v = imgrow[s * nxr + x];
@@ -475,16 +503,13 @@ hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStre
 
 }  // namespace
 
-size_t fused_lds_bytes(int nx, int M, int R, int S, int W)
+size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows)
 {
     const size_t nstrips = (size_t)(nx + R - 1) / R;
     const size_t nxr = nstrips * R;
     const size_t Q = (nxr + 2 * (size_t)M + R) / R;
     const size_t rowlen = Q * R;
-    size_t bytes = 2 * (size_t)S * W * rowlen * sizeof(float2);
-    if (W > 1)
-        bytes += (size_t)S * nxr * sizeof(float);
-    return bytes;
+    return 2 * (size_t)S * W * rowlen * sizeof(float2) + (size_t)img_rows * nxr * sizeof(float);
 }
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,

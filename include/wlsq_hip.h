@@ -113,6 +113,9 @@ typedef struct {
     int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
     int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
                              bit for bit, so only table_forw is read */
+    int image_mode;       /* 0 = auto, 1 = image rows reduced in LDS then one
+                             atomic add per (depth, x) per task, 2 = an atomic
+                             add per point */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

@@ -184,7 +184,8 @@ def test_ktable_fused_matches_oracle(shape):
 @pytest.mark.parametrize("tuning", [
     dict(shots_per_group=1), dict(shots_per_group=2), dict(shots_per_group=4),
     dict(x_per_thread=8), dict(omegas_per_item=1), dict(omegas_per_item=3, block=512),
-    dict(tables_conjugate=1),
+    dict(tables_conjugate=1), dict(image_mode=1), dict(image_mode=2),
+    dict(image_mode=2, shots_per_group=4),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
