@@ -95,6 +95,73 @@ def cpu_baseline(p, nshots_sample=2):
                       f"nx {m.nx} ({pts:.3g} points, {dt:.1f} s, compact-table mode, {cpu})"}
 
 
+def bench_c3(args, log):
+    """128 shots of uneven aperture / frequency count on an 800 x 2048 model in
+    one launch, replayed as a hipGraph; launch shape autotuned on 16 shots."""
+    import torch
+
+    from imaging2d_wlsq_amd import batch, synth
+
+    t0 = time.perf_counter()
+    p = synth.config_c3()
+    sv = batch.Survey(p)
+    tuning = None
+    if not args.no_tune:
+        probe = sv.batch(p.shots[:16], tune=True, reps=1)
+        tuning = probe.tuning
+        probe.close()
+        del probe
+    b = sv.batch(p.shots, tuning=tuning, graph=True)
+    setup = time.perf_counter() - t0
+    b.run()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.c3_steps)]
+    t1 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record()
+        b.run()
+        e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t1) / args.c3_steps
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / len(evs)
+    res = {"workload": p.name, "shots": len(p.shots), "points_per_step": b.points,
+           "value": b.points / (ms / 1e3), "unit": "points/s", "ms_per_step": ms,
+           "wall_ms_per_step": wall * 1e3, "tuning": b.tuning, "plan": b.plan.info(),
+           "graph": True, "setup_s": setup,
+           "achieved_tflops": flops_per_point(p.tables.L) * b.points / (ms / 1e3) / 1e12}
+    log(f"c3: {ms:.1f} ms/step, {res['value']:.3e} points/s, {res['achieved_tflops']:.1f} TFLOP/s")
+    b.close()
+    return res
+
+
+def bench_serving(args, log, service_rate_pts):
+    """Continuous admission of jobs of mixed size against 4 resident models."""
+    import numpy as np
+
+    from imaging2d_wlsq_amd import serve
+
+    t0 = time.perf_counter()
+    models = serve.serving_models()
+    launcher = serve.GpuLauncher(models, args.serve_slots)
+    setup = time.perf_counter() - t0
+    probe = serve.make_jobs(2000, 1.0, len(models), models[0].model.nz, models[0].model.nx, seed=4)
+    mean_pts = float(np.mean([j.points for j in probe]))
+    rate = args.serve_load * service_rate_pts / mean_pts
+    jobs = serve.make_jobs(args.serve_jobs, rate, len(models), models[0].model.nz,
+                           models[0].model.nx, seed=4)
+    budget = 32 << 30
+    st = serve.Server(launcher, hbm_budget=budget).run(jobs)
+    launcher.close()
+    s = st.summary()
+    s.update({"arrival_rate_jobs_per_s": rate, "slots": args.serve_slots, "hbm_budget_bytes": budget,
+              "models": len(models), "model_shape": [models[0].model.nz, models[0].model.nx],
+              "setup_s": setup, "note": f"first {args.serve_jobs} jobs of the seeded 2000-job stream"})
+    log(f"serving: {s['jobs']} jobs, {s['points_per_s']:.3e} points/s, p50 {s['p50_latency_s']:.3f} s, "
+        f"p99 {s['p99_latency_s']:.3f} s, max concurrent {s['max_concurrent_jobs']}")
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +174,14 @@ def main():
     ap.add_argument("--omegas-per-item", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--x-per-thread", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="use the planner's default shape")
+    ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
+    ap.add_argument("--serve-jobs", type=int, default=200,
+                    help="jobs of the serving run (0: skip; the full configuration is 2000)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-load", type=float, default=1.2,
+                    help="arrival rate as a multiple of the measured service rate")
+    ap.add_argument("--tune-verbose", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -119,7 +194,7 @@ def main():
     import numpy as np
     import torch
 
-    from imaging2d_wlsq_amd import device, synth
+    from imaging2d_wlsq_amd import autotune, device, synth
     from imaging2d_wlsq_amd._lib import LAYOUT_TAPMAJOR, lib
 
     lib()
@@ -138,6 +213,7 @@ def main():
     tf = torch.from_numpy(t.forw).to(dev)
     tb = torch.from_numpy(t.back).to(dev)
     conj = device.tables_conjugate(tf, tb)
+    sym = autotune.tables_symmetric(tf) and autotune.tables_symmetric(tb)
     F0, B0, offs = synth.device_fields(p, p.shots, device=dev)
     F0 = F0.view(ns, nf, nx)
     B0 = B0.view(ns, nf, nx)
@@ -154,15 +230,24 @@ def main():
     specs = [device.ShotSpec(f_off=s * nf * nx, b_off=s * nf * nx, img_off=0, slow_off=0, ld=nx,
                              img_ld=nx, slow_ld=nx, nx=nx, nz=nz, nf=nf, omega_off=0)
              for s in range(ns)]
-    tuning = dict(tables_conjugate=int(conj), shots_per_group=args.shots_per_group,
-                  omegas_per_item=args.omegas_per_item, block=args.block,
-                  x_per_thread=args.x_per_thread)
-    plan = device.Plan(specs, t.M, tuning)
-    info = plan.info()
-    log("plan", info)
     F = F0.clone()
     B = B0.clone()
     image = torch.zeros((nz, nx), dtype=torch.float32, device=dev)
+    forced = any([args.shots_per_group, args.omegas_per_item, args.block, args.x_per_thread])
+    if args.no_tune or forced:
+        tuning = dict(tables_conjugate=int(conj), tables_symmetric=int(sym),
+                      shots_per_group=args.shots_per_group, omegas_per_item=args.omegas_per_item,
+                      block=args.block, x_per_thread=args.x_per_thread)
+        tune_s = 0.0
+    else:
+        t_tune = time.perf_counter()
+        tuning = autotune.tune(specs, t.M, slow, omega, t.kappa0, t.dkappa, t.nk, tf, tb, F, B,
+                               image, conj, sym, reps=2, stream=stream, verbose=args.tune_verbose)
+        tune_s = time.perf_counter() - t_tune
+    log("tuning", tuning, f"({tune_s:.1f} s)")
+    plan = device.Plan(specs, t.M, tuning)
+    info = plan.info()
+    log("plan", info)
 
     def fused_step():
         image.zero_()
@@ -209,12 +294,13 @@ def main():
         "config": {"workload": "c2_512x1024_nf256_16shots_L31_nk2048", "nz": nz, "nx": nx,
                    "nf": nf, "shots": ns, "taps": L, "nk": t.nk, "points_per_step": points,
                    "path": "batched fused compact-table kernel (one launch, all shots)",
-                   "plan": info, "tables_conjugate": conj},
+                   "plan": info, "tuning": tuning, "tables_conjugate": conj,
+                   "tables_symmetric": sym},
         "roofline": {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: per-point gather-convolution)",
                      "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                      "traffic": read_profile_traffic("fused_kernel_c2_traffic.txt"),
-                     "kernel": "fused_kernel<31,4,S,conj>", "kernel_ms": k_avg,
+                     "kernel": "fused_kernel (L=31)", "kernel_ms": k_avg,
                      "flops_per_point": flops_per_point(L)},
     }
 
@@ -270,6 +356,13 @@ def main():
                                  "note": "per-launch time = step time / (shots*depths); includes "
                                          "the small copy/reduce launches"}
         del tapF, tapB
+        torch.cuda.empty_cache()
+
+    if args.c3_steps > 0:
+        out["c3_batched"] = bench_c3(args, log)
+        torch.cuda.empty_cache()
+    if args.serve_jobs > 0:
+        out["serving"] = bench_serving(args, log, value)
         torch.cuda.empty_cache()
 
     if not args.no_cpu:

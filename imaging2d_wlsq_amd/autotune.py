@@ -1,0 +1,127 @@
+"""Autotuner for the fused kernel's launch shape.
+
+For a batch of shots it times candidate tunings - shots per group (coefficient
+reuse), x per thread (register blocking), threads per workgroup, frequencies
+per task (LDS rows per workgroup) and the tap-pairing variant when the tables
+allow it - with HIP events on the caller's stream, and returns the fastest.
+Results are cached in memory and, optionally, in a JSON file keyed by the
+workload signature (apertures, depths, frequency counts, taps, device name).
+
+Candidates that the planner rejects (LDS, thread limits) are skipped; every
+candidate computes the same operator, so the choice only affects speed (the
+tap-paired variant changes rounding order; the tests bound it).
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+from typing import Dict, Iterable, List, Optional, Sequence
+
+import torch
+
+from . import device
+from ._lib import WlsqError
+
+_MEM: Dict[str, dict] = {}
+
+
+def signature(shots: Sequence[device.ShotSpec], M: int, conj: bool, sym: bool) -> str:
+    h = hashlib.sha1()
+    for s in shots:
+        h.update(f"{s.nx},{s.nz},{s.nf},{s.omega_off},{s.slow_off},{int(s.img_off)};".encode())
+    name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    return f"{name}|M{M}|c{int(conj)}|s{int(sym)}|n{len(shots)}|{h.hexdigest()[:16]}"
+
+
+def candidates(conj: bool, sym: bool, max_nx: int) -> List[dict]:
+    out = []
+    syms = [1, 0] if sym else [0]
+    for sy in syms:
+        for S in (1, 2, 4):
+            for R in (2, 4, 8):
+                if R == 8 and S != 1:
+                    continue
+                if (max_nx + R - 1) // R > (1024 if R == 8 else 512):
+                    continue
+                if R == 2 and S == 1:
+                    continue
+                blocks = [256, 512] if R == 4 else [0]
+                for block in blocks:
+                    if block and block < (max_nx + R - 1) // R:
+                        continue
+                    for W in (0, 1):
+                        out.append(dict(shots_per_group=S, x_per_thread=R, block=block,
+                                        omegas_per_item=W, tables_conjugate=int(conj),
+                                        tables_symmetric=sy))
+    return out
+
+
+def _time_plan(plan: device.Plan, args, reps: int, stream) -> float:
+    slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image = args
+    plan.launch(slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image, stream=stream)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        plan.launch(slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image, stream=stream)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return min(e0.elapsed_time(e1) for e0, e1 in ev)
+
+
+def tune(shots: Sequence[device.ShotSpec], M: int, slowness, omega, kappa0, dkappa, nk, tf, tb,
+         F, B, image, conj: bool, sym: bool, reps: int = 3, cache_file: Optional[str] = None,
+         cands: Optional[Iterable[dict]] = None, stream=None, verbose=False) -> dict:
+    """Best tuning dict for this batch (``image`` is used as scratch and
+    left holding garbage; F/B are not written back)."""
+    key = signature(shots, M, conj, sym)
+    if key in _MEM:
+        return dict(_MEM[key])
+    if cache_file and os.path.exists(cache_file):
+        try:
+            disk = json.load(open(cache_file))
+            if key in disk:
+                _MEM[key] = disk[key]["tuning"]
+                return dict(_MEM[key])
+        except (OSError, ValueError):
+            pass
+    stream = stream or torch.cuda.current_stream()
+    max_nx = max(s.nx for s in shots)
+    results = []
+    args = (slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image)
+    for c in (cands or candidates(conj, sym, max_nx)):
+        try:
+            plan = device.Plan(shots, M, c)
+        except WlsqError:
+            continue
+        try:
+            ms = _time_plan(plan, args, reps, stream)
+        except WlsqError:
+            plan.close()
+            continue
+        info = plan.info()
+        plan.close()
+        results.append((ms, c, info))
+        if verbose:
+            print(f"  tune {c} -> {ms:.3f} ms {info}", flush=True)
+    if not results:
+        raise RuntimeError("no launch shape fits this batch")
+    results.sort(key=lambda r: r[0])
+    best = results[0][1]
+    _MEM[key] = best
+    if cache_file:
+        try:
+            disk = json.load(open(cache_file)) if os.path.exists(cache_file) else {}
+        except (OSError, ValueError):
+            disk = {}
+        disk[key] = {"tuning": best, "ms": results[0][0]}
+        with open(cache_file, "w") as f:
+            json.dump(disk, f, indent=1)
+    return dict(best)
+
+
+def tables_symmetric(table: torch.Tensor) -> bool:
+    """True when table[q][k] == table[q][L-1-k] bit for bit for every row."""
+    v = torch.view_as_real(table).contiguous().view(torch.int32)
+    return bool(torch.equal(v, torch.flip(v, dims=[1])))

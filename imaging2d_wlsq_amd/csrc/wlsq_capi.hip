@@ -221,7 +221,7 @@ int transpose_tables(Scratch &sc, int nextrap, int nf, int nx, int M, const floa
 struct PlanImpl {
     int M = 0;
     int L = 1;
-    FusedShape shape{4, 1, 256, false};
+    FusedShape shape{4, 1, 1, 256, false, false};
     size_t lds = 0;
     int img_lds = 1;
     long long points = 0;
@@ -280,13 +280,15 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     int R = t.x_per_thread;
     if (R == 0)
         R = (max_nx + 3) / 4 <= 512 ? 4 : 8;
-    if (R != 4 && R != 8)
-        return fail(WLSQ_EINVAL, "x_per_thread must be 4 or 8 (got %d)", R);
-    const int max_block = R == 8 ? 1024 : 512;
+    if (R != 2 && R != 4 && R != 8)
+        return fail(WLSQ_EINVAL, "x_per_thread must be 2, 4 or 8 (got %d)", R);
+    const int max_block = 512;
     const int max_strips = (max_nx + R - 1) / R;
-    if (max_strips > max_block)
-        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d threads at %d x per thread (max %d)",
-                    max_nx, max_strips, R, max_block);
+    // strips per thread: 2 only with 8 x per thread (apertures up to 8192)
+    const int C = max_strips <= max_block ? 1 : 2;
+    if (max_strips > C * max_block || (C == 2 && R != 8))
+        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
+                    max_strips, R, (R == 8 ? 2 : 1) * max_block);
     int S = t.shots_per_group;
     if (S == 0) {
         int run = 1, best = 1;
@@ -294,20 +296,22 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
             run = same_shape(shots[i], shots[i - 1]) ? run + 1 : 1;
             best = std::max(best, run);
         }
-        S = R == 8 ? 1 : (best >= 4 ? 4 : best >= 2 ? 2 : 1);
+        S = R == 8 ? 1 : R == 2 ? (best >= 4 ? 4 : 2) : (best >= 4 ? 4 : best >= 2 ? 2 : 1);
     }
     if (S != 1 && S != 2 && S != 4)
         return fail(WLSQ_EINVAL, "shots_per_group must be 1, 2 or 4 (got %d)", S);
     if (R == 8 && S != 1)
         return fail(WLSQ_EUNSUPPORTED, "8 x per thread supports 1 shot per group");
+    if (R == 2 && S == 1)
+        return fail(WLSQ_EUNSUPPORTED, "2 x per thread supports 2 or 4 shots per group");
 
     int block = t.block;
     if (block == 0) {
-        block = ((max_strips + 63) / 64) * 64;
+        block = (((max_strips + C - 1) / C + 63) / 64) * 64;
         if (block < 256)
-            block = std::min(256, max_block);
+            block = 256;
     }
-    if (block < 64 || block > max_block || block % 64 || block < max_strips)
+    if (block < 64 || block > max_block || block % 64 || block * C < max_strips)
         return fail(WLSQ_EINVAL, "block=%d invalid for %d strips (multiple of 64, <= %d)", block,
                     max_strips, max_block);
 
@@ -380,7 +384,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         g.pad_ = 0;
         const int img_rows = img_lds ? (g.img_shared ? 1 : S) : 0;
         const int strips = (s.nx + R - 1) / R;
-        int W = block / strips;
+        int W = block * C / strips;
         if (t.omegas_per_item > 0)
             W = std::min(W, t.omegas_per_item);
         W = std::min(W, s.nf);
@@ -413,7 +417,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         pend[i].item.cost_rank = (int)i;
         P->items.push_back(pend[i].item);
     }
-    P->shape = FusedShape{R, S, block, t.tables_conjugate != 0};
+    P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0};
     P->lds = lds_need;
     P->img_lds = img_lds;
     if (!P->groups.empty()) {
@@ -503,7 +507,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
         t = *tuning;
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
-                               t.tables_conjugate, t.image_mode};
+                               t.tables_conjugate, t.image_mode, t.tables_symmetric};
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -521,6 +525,15 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
     }
     return launch_plan(*P, slowness, omega, kappa0, dkappa, nk, tf, tb, forw, back, image,
                        write_back, st);
+}
+
+bool host_tables_symmetric(const wlsq_c32 *t, int nk, int L)
+{
+    for (long long q = 0; q < nk; ++q)
+        for (int k = 0; k < L / 2; ++k)
+            if (std::memcmp(&t[q * L + k], &t[q * L + L - 1 - k], sizeof(wlsq_c32)) != 0)
+                return false;
+    return true;
 }
 
 bool host_tables_conjugate(const wlsq_c32 *tf, const wlsq_c32 *tb, long long n)
@@ -733,6 +746,8 @@ int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
     }
     wlsq_tuning t{};
     t.tables_conjugate = host_tables_conjugate(table_forw, table_back, (long long)nk * L) ? 1 : 0;
+    t.tables_symmetric =
+        host_tables_symmetric(table_forw, nk, L) && host_tables_symmetric(table_back, nk, L) ? 1 : 0;
     // device layout of the fields: (ns, nf, nx), i.e. "nt" = nf; image (ns, nextrap, nx)
     WLSQ_TRY(ktable_device_impl(d, ns, nextrap, nextrap, nf, nf, nx, M, sc.slow.as<float>(), nx,
                                 sc.omega.as<float>(), kappa0, dkappa, nk,

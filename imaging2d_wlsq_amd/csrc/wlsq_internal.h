@@ -59,10 +59,12 @@ struct FusedArgs {
 
 // Launch shape of the fused kernel; chosen by the planner.
 struct FusedShape {
-    int R;      // consecutive x per thread (4 or 8)
+    int R;      // consecutive x per thread (2, 4, 8)
     int S;      // shots per group (1, 2, 4)
+    int C;      // strips per thread (1, or 2 with R = 8)
     int block;  // threads per workgroup
     bool conj;  // table_back == conj(table_forw): load one table
+    bool sym;   // tables are tap-symmetric (T[k] == T[L-1-k]): pair the taps
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of

@@ -61,9 +61,11 @@ __device__ __forceinline__ int swz(int p, int Q)
 }
 
 // ------------------------------------------------------------------ fused kernel
-template <int L_, int R, int S, bool CONJ, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, int C, int MAXT>
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
+    constexpr int LOG2R = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
+    static_assert((1 << LOG2R) == R, "R must be a power of two");
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int NT = blockDim.x;
@@ -82,7 +84,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const int rowlen = Q * R;
     const int W = it.nw;
     const int nrows = S * W;
-    if (W * nstrips > NT)
+    if (W * nstrips > C * NT)
         return;  // the planner never builds such a task
     float2 *rowsF = lds;
     float2 *rowsB = lds + (size_t)nrows * rowlen;
@@ -121,19 +123,33 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const float2 *__restrict__ TF = a.TF;
     const float2 *__restrict__ TB = a.TB;
 
-    const bool active = tid < W * nstrips;
-    const int w = active ? tid / nstrips : 0;
-    const int strip = active ? tid - w * nstrips : 0;
-    const int x0 = strip * R;
-    const float om = active ? a.omega[gp->omega_off + it.j0 + w] : 0.f;
-    const float2 *rF = rowsF + (size_t)w * rowlen;   // shot s adds s*W*rowlen
-    const float2 *rB = rowsB + (size_t)w * rowlen;
+    // Each thread owns C strips (task slots tid, tid + NT, ...): slot ->
+    // (frequency row w, strip of R consecutive x).
     const size_t shot_rows = (size_t)W * rowlen;
+    bool actv[C];
+    int wv[C], x0v[C];
+    float omv[C];
+#pragma unroll
+    for (int cs = 0; cs < C; ++cs) {
+        const int slot = tid + cs * NT;
+        actv[cs] = slot < W * nstrips;
+        wv[cs] = actv[cs] ? slot / nstrips : 0;
+        x0v[cs] = actv[cs] ? (slot - wv[cs] * nstrips) * R : 0;
+        omv[cs] = actv[cs] ? a.omega[gp->omega_off + it.j0 + wv[cs]] : 0.f;
+    }
     __syncthreads();
 
-    float2 accF[S][R], accB[S][R];
+    float2 accF_[C][S][R], accB_[C][S][R];
     for (int l = 0; l < nz; ++l) {
-        if (active) {
+#pragma unroll
+        for (int cs = 0; cs < C; ++cs) {
+        if (actv[cs]) {
+            float2 (&accF)[S][R] = accF_[cs];
+            float2 (&accB)[S][R] = accB_[cs];
+            const int x0 = x0v[cs];
+            const float om = omv[cs];
+            const float2 *rF = rowsF + (size_t)wv[cs] * rowlen;  // shot s adds s*W*rowlen
+            const float2 *rB = rowsB + (size_t)wv[cs] * rowlen;
             const float *srow = slow0 + (long long)l * slow_ld;
             int qo[R];
 #pragma unroll
@@ -141,68 +157,162 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 const int xr = min(x0 + r, nx - 1);
                 qo[r] = kappa_index(om, srow[xr], a.kappa0, a.inv_dk, a.nk) * L;
             }
-            float2 wF[S][R], wB[S][R];
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int o = swz<R>(x0 + r, Q);
-                    wF[s][r] = rF[s * shot_rows + o];
-                    wB[s][r] = rB[s * shot_rows + o];
                     accF[s][r] = make_float2(0.f, 0.f);
                     accB[s][r] = make_float2(0.f, 0.f);
                 }
             }
-            // Taps in chunks of R: after R window shifts the registers are back
-            // in place, so the chunk loop needs no moves; one chunk's R*R
-            // coefficient loads per table are in flight at a time.
-            auto tap = [&](const int k) {
+            // Row position p of this thread's rows lives at sub-row (p mod R),
+            // slot p / R; x0 is a multiple of R, so p = x0 + j is at sub-row
+            // (j mod R), slot xs + j / R.
+            const int xs = x0 / R;
+            auto ldF = [&](int s, int j) { return rF[s * shot_rows + (j & (R - 1)) * Q + xs + (j >> LOG2R)]; };
+            auto ldB = [&](int s, int j) { return rB[s * shot_rows + (j & (R - 1)) * Q + xs + (j >> LOG2R)]; };
+            if (!SYM) {
+                // new[x] = sum_k T[k] old[x + k - M]: a window of R inputs slides
+                // right by one per tap. Taps go in chunks of R so the window
+                // registers are back in place after each chunk.
+                float2 wF[S][R], wB[S][R];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        wF[s][r] = ldF(s, r);
+                        wB[s][r] = ldB(s, r);
+                    }
+                }
+                auto tap = [&](const int k) {
+                    float2 cf[R], cb[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        cf[r] = TF[qo[r] + k];
+                        cb[r] = CONJ ? make_float2(cf[r].x, -cf[r].y) : TB[qo[r] + k];
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            accF[s][r] = cmac(accF[s][r], cf[r], wF[s][r]);
+                            accB[s][r] = cmac(accB[s][r], cb[r], wB[s][r]);
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r + 1 < R; ++r) {
+                            wF[s][r] = wF[s][r + 1];
+                            wB[s][r] = wB[s][r + 1];
+                        }
+                        wF[s][R - 1] = ldF(s, R + k);  // the slot past the last tap is padding
+                        wB[s][R - 1] = ldB(s, R + k);
+                    }
+                    // wide windows: keep the next tap's R coefficient loads from
+                    // being hoisted over this one (bounds live registers)
+                    if constexpr (R >= 8)
+                        __builtin_amdgcn_sched_barrier(0);
+                };
+                const int nch = L / R;
+#pragma unroll 1
+                for (int c = 0; c < nch; ++c) {
+#pragma unroll
+                    for (int r2 = 0; r2 < R; ++r2)
+                        tap(c * R + r2);
+                }
+                if (L_ > 0) {
+#pragma unroll
+                    for (int k = (L_ / R) * R; k < L_; ++k)
+                        tap(k);
+                } else {
+#pragma unroll 1
+                    for (int k = nch * R; k < L; ++k)
+                        tap(k);
+                }
+            } else {
+                // Tap-symmetric operator (T[k] == T[L-1-k]):
+                //   new[x] = T[M] old[x] + sum_{k<M} T[k] (old[x+k-M] + old[x+M-k]).
+                // A left window slides right and a right window slides left;
+                // after the M pairs the left window holds old[x] for the centre.
+                float2 lF[S][R], lB[S][R], rFw[S][R], rBw[S][R];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        lF[s][r] = ldF(s, r);
+                        lB[s][r] = ldB(s, r);
+                        rFw[s][r] = ldF(s, r + 2 * M);
+                        rBw[s][r] = ldB(s, r + 2 * M);
+                    }
+                }
+                auto pair = [&](const int k) {
+                    float2 cf[R], cb[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        cf[r] = TF[qo[r] + k];
+                        cb[r] = CONJ ? make_float2(cf[r].x, -cf[r].y) : TB[qo[r] + k];
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const float2 sf = make_float2(lF[s][r].x + rFw[s][r].x, lF[s][r].y + rFw[s][r].y);
+                            const float2 sb = make_float2(lB[s][r].x + rBw[s][r].x, lB[s][r].y + rBw[s][r].y);
+                            accF[s][r] = cmac(accF[s][r], cf[r], sf);
+                            accB[s][r] = cmac(accB[s][r], cb[r], sb);
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r + 1 < R; ++r) {
+                            lF[s][r] = lF[s][r + 1];
+                            lB[s][r] = lB[s][r + 1];
+                        }
+                        lF[s][R - 1] = ldF(s, R + k);
+                        lB[s][R - 1] = ldB(s, R + k);
+#pragma unroll
+                        for (int r = R - 1; r > 0; --r) {
+                            rFw[s][r] = rFw[s][r - 1];
+                            rBw[s][r] = rBw[s][r - 1];
+                        }
+                        rFw[s][0] = ldF(s, 2 * M - 1 - k);
+                        rBw[s][0] = ldB(s, 2 * M - 1 - k);
+                    }
+                    if constexpr (R >= 8)
+                        __builtin_amdgcn_sched_barrier(0);
+                };
+                const int nch = M / R;
+#pragma unroll 1
+                for (int c = 0; c < nch; ++c) {
+#pragma unroll
+                    for (int r2 = 0; r2 < R; ++r2)
+                        pair(c * R + r2);
+                }
+                if (L_ > 0) {
+#pragma unroll
+                    for (int k = (((L_ - 1) / 2) / R) * R; k < (L_ - 1) / 2; ++k)
+                        pair(k);
+                } else {
+#pragma unroll 1
+                    for (int k = nch * R; k < M; ++k)
+                        pair(k);
+                }
                 float2 cf[R], cb[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    cf[r] = TF[qo[r] + k];
-                    if (CONJ)
-                        cb[r] = make_float2(cf[r].x, -cf[r].y);
-                    else
-                        cb[r] = TB[qo[r] + k];
+                    cf[r] = TF[qo[r] + M];
+                    cb[r] = CONJ ? make_float2(cf[r].x, -cf[r].y) : TB[qo[r] + M];
                 }
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        accF[s][r] = cmac(accF[s][r], cf[r], wF[s][r]);
-                        accB[s][r] = cmac(accB[s][r], cb[r], wB[s][r]);
+                        accF[s][r] = cmac(accF[s][r], cf[r], lF[s][r]);
+                        accB[s][r] = cmac(accB[s][r], cb[r], lB[s][r]);
                     }
                 }
-                // slide the window by one (the value past the last tap is a
-                // zero-padded slot inside the row)
-                const int o = swz<R>(x0 + R + k, Q);
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-#pragma unroll
-                    for (int r = 0; r + 1 < R; ++r) {
-                        wF[s][r] = wF[s][r + 1];
-                        wB[s][r] = wB[s][r + 1];
-                    }
-                    wF[s][R - 1] = rF[s * shot_rows + o];
-                    wB[s][R - 1] = rB[s * shot_rows + o];
-                }
-            };
-            const int Lc = (L / R) * R;
-#pragma unroll 1
-            for (int kb = 0; kb < Lc; kb += R) {
-#pragma unroll
-                for (int r2 = 0; r2 < R; ++r2)
-                    tap(kb + r2);
-            }
-            if (L_ > 0) {
-#pragma unroll
-                for (int k = (L_ / R) * R; k < L_; ++k)
-                    tap(k);
-            } else {
-#pragma unroll 1
-                for (int k = Lc; k < L; ++k)
-                    tap(k);
             }
             // imaging condition of this step. With img_lds the task's rows are
             // summed in LDS (over its frequencies, and over its shots when they
@@ -246,17 +356,21 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 }
             }
         }
+        }
         __syncthreads();
-        if (active) {
+#pragma unroll
+        for (int cs = 0; cs < C; ++cs) {
+            if (!actv[cs])
+                continue;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int x = x0 + r;
+                    const int x = x0v[cs] + r;
                     if (x < nx) {
                         const int o = swz<R>(x + M, Q);
-                        rowsF[s * shot_rows + (size_t)w * rowlen + o] = accF[s][r];
-                        rowsB[s * shot_rows + (size_t)w * rowlen + o] = accB[s][r];
+                        rowsF[s * shot_rows + (size_t)wv[cs] * rowlen + o] = accF_[cs][s][r];
+                        rowsB[s * shot_rows + (size_t)wv[cs] * rowlen + o] = accB_[cs][s][r];
                     }
                 }
             }
@@ -470,10 +584,10 @@ hipError_t ensure_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int L_, int R, int S, bool CONJ, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, int C, int MAXT>
 hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
 {
-    auto k = fused_kernel<L_, R, S, CONJ, MAXT>;
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, C, MAXT>;
     hipError_t e = ensure_lds(k, lds);
     if (e != hipSuccess)
         return e;
@@ -481,23 +595,39 @@ hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, co
     return hipGetLastError();
 }
 
+template <int L_, int R, int S, int C, int MAXT>
+hipError_t launch_fused_RS(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
+                           const FusedArgs &a)
+{
+    const int b = sh.block;
+    if (sh.sym)
+        return sh.conj ? launch_fused_t<L_, R, S, true, true, C, MAXT>(n_items, b, lds, st, a)
+                       : launch_fused_t<L_, R, S, false, true, C, MAXT>(n_items, b, lds, st, a);
+    return sh.conj ? launch_fused_t<L_, R, S, true, false, C, MAXT>(n_items, b, lds, st, a)
+                   : launch_fused_t<L_, R, S, false, false, C, MAXT>(n_items, b, lds, st, a);
+}
+
 template <int L_>
 hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
                           const FusedArgs &a)
 {
-    const int b = sh.block;
-    if (sh.R == 4 && sh.S == 1)
-        return sh.conj ? launch_fused_t<L_, 4, 1, true, 512>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, 4, 1, false, 512>(n_items, b, lds, st, a);
-    if (sh.R == 4 && sh.S == 2)
-        return sh.conj ? launch_fused_t<L_, 4, 2, true, 512>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, 4, 2, false, 512>(n_items, b, lds, st, a);
-    if (sh.R == 4 && sh.S == 4)
-        return sh.conj ? launch_fused_t<L_, 4, 4, true, 512>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, 4, 4, false, 512>(n_items, b, lds, st, a);
-    if (sh.R == 8 && sh.S == 1)
-        return sh.conj ? launch_fused_t<L_, 8, 1, true, 1024>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, 8, 1, false, 1024>(n_items, b, lds, st, a);
+    if (sh.C == 1) {
+        if (sh.R == 4 && sh.S == 1)
+            return launch_fused_RS<L_, 4, 1, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 4 && sh.S == 2)
+            return launch_fused_RS<L_, 4, 2, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 4 && sh.S == 4)
+            return launch_fused_RS<L_, 4, 4, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 8 && sh.S == 1)
+            return launch_fused_RS<L_, 8, 1, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 2 && sh.S == 2)
+            return launch_fused_RS<L_, 2, 2, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 2 && sh.S == 4)
+            return launch_fused_RS<L_, 2, 4, 1, 512>(sh, n_items, lds, st, a);
+    } else if (sh.C == 2) {
+        if (sh.R == 8 && sh.S == 1)
+            return launch_fused_RS<L_, 8, 1, 2, 512>(sh, n_items, lds, st, a);
+    }
     return hipErrorInvalidValue;
 }
 
@@ -517,7 +647,7 @@ hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hip
 {
     if (n_items <= 0)
         return hipSuccess;
-    if (sh.block <= 0 || sh.block > (sh.R == 8 ? 1024 : 512) || sh.block % 64 != 0)
+    if (sh.block <= 0 || sh.block > 512 || sh.block % 64 != 0)
         return hipErrorInvalidValue;
     switch (a.L) {
     case 25: return launch_fused_L<25>(sh, n_items, lds_bytes, st, a);

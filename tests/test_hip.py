@@ -154,6 +154,8 @@ KT_SHAPES = [  # ns, nextrap, nf, nx, M, conj
     (4, 8, 9, 1024, 15, True),
     (1, 4, 4, 300, 25, True),
     (2, 3, 2, 2100, 15, True),
+    (1, 3, 2, 4100, 15, True),
+    (1, 2, 2, 8192, 25, True),
     (1, 2, 3, 40, 21, False),
 ]
 
@@ -185,7 +187,7 @@ def test_ktable_fused_matches_oracle(shape):
     dict(shots_per_group=1), dict(shots_per_group=2), dict(shots_per_group=4),
     dict(x_per_thread=8), dict(omegas_per_item=1), dict(omegas_per_item=3, block=512),
     dict(tables_conjugate=1), dict(image_mode=1), dict(image_mode=2),
-    dict(image_mode=2, shots_per_group=4),
+    dict(image_mode=2, shots_per_group=4), dict(x_per_thread=2, shots_per_group=4),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
@@ -215,6 +217,45 @@ def test_ktable_device_tunings_agree(tuning):
     np.testing.assert_array_equal(outs[1][2], outs[0][2])
     assert rel(outs[1][0], outs[0][0]) <= 1e-6
     assert rel(outs[1][0], io) <= RTOL
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(shots_per_group=1), dict(shots_per_group=2), dict(shots_per_group=4),
+    dict(x_per_thread=8), dict(omegas_per_item=3, block=512), dict(tables_conjugate=1),
+    dict(x_per_thread=2, shots_per_group=2), dict(x_per_thread=2, shots_per_group=4),
+], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
+@pytest.mark.parametrize("M", [15, 4, 0, 25])
+def test_tap_paired_variant_matches_oracle(tuning, M):
+    """Tap-symmetric tables (the WLSQ design gives them): the paired-tap
+    kernel computes the same operator in a different rounding order."""
+    import torch
+    from imaging2d_wlsq_amd import autotune, device
+    rng = np.random.default_rng(400 + M)
+    ns, nextrap, nf, nx = 5, 6, 7, 301
+    slow, omega, k0, dk, tf, _ = _ktable_problem(rng, nx, M, nf, nextrap, conj=True)
+    tf = np.ascontiguousarray((tf + tf[:, ::-1]) * 0.5).astype(np.complex64)
+    tb = np.conj(tf).astype(np.complex64)
+    assert autotune.tables_symmetric(torch.from_numpy(tf).cuda())
+    F = _rand(rng, (ns, nf, nx))
+    B = _rand(rng, (ns, nf, nx))
+    Fo, Bo = F.copy(), B.copy()
+    io = np.zeros((ns, nextrap, nx), np.float32)
+    oracle.extrapolate_ktable(ns, nextrap, nextrap, nx, nf, nf, M, slow, omega, k0, dk, 300, tf, tb, Fo, Bo, io)
+    outs = []
+    for sym in (0, 1):
+        t = dict(tuning, tables_symmetric=sym)
+        print("tuning", t, "M", M, flush=True)
+        dF, dB = torch.from_numpy(F).cuda(), torch.from_numpy(B).cuda()
+        img = torch.zeros((ns, nextrap, nx), device="cuda")
+        device.ktable_device(ns, nextrap, nextrap, nf, nf, nx, M, torch.from_numpy(slow).cuda(),
+                             torch.from_numpy(omega).cuda(), k0, dk, 300, torch.from_numpy(tf).cuda(),
+                             dF, torch.from_numpy(tb).cuda(), dB, img, tuning=t)
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), dF.cpu().numpy(), dB.cpu().numpy()))
+    for o in outs:
+        assert rel(o[0], io) <= RTOL
+        assert crel(o[1], Fo) <= RTOL and crel(o[2], Bo) <= RTOL
+    assert crel(outs[1][1], outs[0][1]) <= 1e-6
 
 
 @pytest.mark.parametrize("layout", [0, 1])
@@ -407,9 +448,11 @@ def test_errors_are_reported_not_silent():
     rng = np.random.default_rng(1)
     b = _batch_setup(rng)
     specs = _batch_specs(b)
-    specs[0].nx = 5000  # aperture beyond what 8 x per thread x 1024 threads covers? (5000/8 = 625 ok)
+    specs[0].nx = 5000
     specs[0].ld = 5000
     specs[0].img_ld = 5000
     specs[0].slow_ld = 5000
     with pytest.raises(_lib.WlsqError):
         device.Plan(specs, b["M"], dict(x_per_thread=4))  # 1250 strips > 512 threads
+    with pytest.raises(_lib.WlsqError):
+        device.Plan(specs, b["M"], dict(x_per_thread=8, shots_per_group=2))
