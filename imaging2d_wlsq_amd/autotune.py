@@ -51,9 +51,10 @@ def candidates(conj: bool, sym: bool, max_nx: int) -> List[dict]:
                     if block and block < (max_nx + R - 1) // R:
                         continue
                     for W in (0, 1):
-                        out.append(dict(shots_per_group=S, x_per_thread=R, block=block,
-                                        omegas_per_item=W, tables_conjugate=int(conj),
-                                        tables_symmetric=sy))
+                        for tl in ((1, 2) if S <= 2 and R <= 4 else (1,)):
+                            out.append(dict(shots_per_group=S, x_per_thread=R, block=block,
+                                            omegas_per_item=W, tables_conjugate=int(conj),
+                                            tables_symmetric=sy, tap_loads=tl))
     return out
 
 

@@ -21,12 +21,19 @@ import torch
 from . import autotune, device, synth
 
 
-def default_tuning(max_nx: int, shots_share: bool, conj: bool, sym: bool) -> dict:
-    """Launch shape without timing: the shapes the tuner picks most often."""
+def default_tuning(max_nx: int, shots_share: bool, conj: bool, sym: bool, nshots: int = 1) -> dict:
+    """Launch shape without timing, from an autotuner sweep on MI355X over
+    apertures 512-8192 and 1-16 shots per group (tap pairing wins whenever
+    the tables allow it; two shots per group at 2 x per thread for apertures
+    up to 1024, 4 shots for large groups)."""
+    base = dict(tables_conjugate=int(conj), tables_symmetric=int(sym))
     if max_nx > 2048:
-        return dict(x_per_thread=8, shots_per_group=1, tables_conjugate=int(conj))
-    return dict(x_per_thread=4, shots_per_group=4 if shots_share else 1,
-                tables_conjugate=int(conj))
+        return dict(base, x_per_thread=8, shots_per_group=1)
+    if not shots_share:
+        return dict(base, x_per_thread=4, shots_per_group=1, block=512, omegas_per_item=1)
+    if max_nx <= 1024:
+        return dict(base, x_per_thread=2, shots_per_group=4 if nshots >= 8 else 2)
+    return dict(base, x_per_thread=4, shots_per_group=2, block=512)
 
 
 class Survey:
@@ -95,7 +102,8 @@ class Batch:
                                             sv.conj, sv.sym, reps=self.reps)
             else:
                 share = len({(s.nx, s.x0, s.nf, s.nz) for s in self.shots}) < len(self.shots)
-                self.tuning = default_tuning(max(s.nx for s in self.shots), share, sv.conj, sv.sym)
+                self.tuning = default_tuning(max(s.nx for s in self.shots), share, sv.conj, sv.sym,
+                                             len(self.shots))
         self.plan = device.Plan(self.specs, t.M, self.tuning)
         self._graph = None
         if self.graph:

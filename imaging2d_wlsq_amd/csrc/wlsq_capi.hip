@@ -221,7 +221,7 @@ int transpose_tables(Scratch &sc, int nextrap, int nf, int nx, int M, const floa
 struct PlanImpl {
     int M = 0;
     int L = 1;
-    FusedShape shape{4, 1, 1, 256, false, false};
+    FusedShape shape{4, 1, 1, 256, false, false, false};
     size_t lds = 0;
     int img_lds = 1;
     long long points = 0;
@@ -229,6 +229,8 @@ struct PlanImpl {
     std::vector<Item> items;
     Group *d_groups = nullptr;
     Item *d_items = nullptr;
+    float2 *d_tt = nullptr;  // row-padded table copies: forward at 0, backward at Lp * tt_nk
+    int tt_nk = 0;
     int device = 0;
     ~PlanImpl()
     {
@@ -236,6 +238,23 @@ struct PlanImpl {
             (void)hipFree(d_groups);
         if (d_items)
             (void)hipFree(d_items);
+        if (d_tt)
+            (void)hipFree(d_tt);
+    }
+    // table copy storage for nk samples (allocates: call outside graph capture)
+    int reserve_tables(int nk)
+    {
+        if (nk <= tt_nk)
+            return WLSQ_OK;
+        if (d_tt)
+            (void)hipFree(d_tt);
+        d_tt = nullptr;
+        tt_nk = 0;
+        hipError_t e = hipMalloc(&d_tt, sizeof(float2) * 2 * (size_t)(L + 1) * nk);
+        if (e != hipSuccess)
+            return fail(WLSQ_ENOMEM, "hipMalloc(table copies): %s", hipGetErrorString(e));
+        tt_nk = nk;
+        return WLSQ_OK;
     }
 };
 
@@ -417,7 +436,11 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         pend[i].item.cost_rank = (int)i;
         P->items.push_back(pend[i].item);
     }
-    P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0};
+    if (t.tap_loads < 0 || t.tap_loads > 2)
+        return fail(WLSQ_EINVAL, "tap_loads must be 0, 1 or 2 (got %d)", t.tap_loads);
+    // paired tap loads relieve the address unit when no shots share the gathers
+    const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4);
+    P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
     P->lds = lds_need;
     P->img_lds = img_lds;
     if (!P->groups.empty()) {
@@ -434,7 +457,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     return WLSQ_OK;
 }
 
-int launch_plan(const PlanImpl &P, const float *slowness, const float *omega, float kappa0,
+int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float kappa0,
                 float dkappa, int nk, const wlsq_c32 *tf, const wlsq_c32 *tb, wlsq_c32 *forw,
                 wlsq_c32 *back, float *image, int write_back, hipStream_t st)
 {
@@ -444,6 +467,12 @@ int launch_plan(const PlanImpl &P, const float *slowness, const float *omega, fl
         return fail(WLSQ_EINVAL, "NULL device pointer");
     if (nk <= 0 || !(dkappa > 0.0f))
         return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    WLSQ_TRY(P.reserve_tables(nk));
+    float2 *ttF = P.d_tt;
+    float2 *ttB = P.d_tt + (size_t)(P.L + 1) * nk;
+    HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tf), ttF, nk, P.L, st));
+    if (!P.shape.conj)
+        HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tb), ttB, nk, P.L, st));
     FusedArgs a{};
     a.items = P.d_items;
     a.groups = P.d_groups;
@@ -452,8 +481,8 @@ int launch_plan(const PlanImpl &P, const float *slowness, const float *omega, fl
     a.img = image;
     a.omega = omega;
     a.slow = slowness;
-    a.TF = reinterpret_cast<const float2 *>(tf);
-    a.TB = reinterpret_cast<const float2 *>(tb ? tb : tf);
+    a.TF = ttF;
+    a.TB = P.shape.conj ? ttF : ttB;
     a.nk = nk;
     a.kappa0 = kappa0;
     a.inv_dk = inv_dkappa(dkappa);
@@ -507,7 +536,8 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
         t = *tuning;
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
-                               t.tables_conjugate, t.image_mode, t.tables_symmetric};
+                               t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
+                               (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -875,6 +905,8 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
     *out = nullptr;
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
+    if (nk > 0)
+        WLSQ_TRY(plan->impl->reserve_tables(nk));  // no allocation inside the capture
     hipStream_t cap = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     auto g = std::make_unique<wlsq_graph_s>();

@@ -47,7 +47,7 @@ struct FusedArgs {
     float *img;
     const float *omega;
     const float *slow;
-    const float2 *TF;
+    const float2 *TF;  // row-padded copies (nk, L + 1) of the compact tables
     const float2 *TB;
     int nk;
     float kappa0;
@@ -65,6 +65,7 @@ struct FusedShape {
     int block;  // threads per workgroup
     bool conj;  // table_back == conj(table_forw): load one table
     bool sym;   // tables are tap-symmetric (T[k] == T[L-1-k]): pair the taps
+    bool paired;  // gather two taps per 16-byte load (fewer address-unit cycles)
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of
@@ -93,6 +94,8 @@ hipError_t launch_expand(const float *slow, int slow_ld, const float *omega, flo
                          float inv_dk, int nk, const float2 *table, float2 *out, int nextrap,
                          int nf, int nx, int L, int layout, hipStream_t st);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+// compact (nk, L) table -> rows of L + 1 entries (last 0), read by the fused kernel
+hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStream_t st);
 
 // (float)(1.0 / (double)dkappa): the reciprocal every implementation of the
 // kappa lookup uses.

@@ -61,7 +61,7 @@ __device__ __forceinline__ int swz(int p, int Q)
 }
 
 // ------------------------------------------------------------------ fused kernel
-template <int L_, int R, int S, bool CONJ, bool SYM, int C, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
     constexpr int LOG2R = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
@@ -120,8 +120,11 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const int img_ld = gp->img_ld;
     const int slow_ld = gp->slow_ld;
     const float *slow0 = a.slow + gp->slow_off;
+    // tables have rows of Lp = L + 1 entries (16-byte aligned): taps are
+    // gathered in pairs, one 16-byte load per pair and table
     const float2 *__restrict__ TF = a.TF;
     const float2 *__restrict__ TB = a.TB;
+    const int Lp = L + 1;
 
     // Each thread owns C strips (task slots tid, tid + NT, ...): slot ->
     // (frequency row w, strip of R consecutive x).
@@ -155,7 +158,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int xr = min(x0 + r, nx - 1);
-                qo[r] = kappa_index(om, srow[xr], a.kappa0, a.inv_dk, a.nk) * L;
+                qo[r] = kappa_index(om, srow[xr], a.kappa0, a.inv_dk, a.nk) * Lp;
             }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
@@ -184,13 +187,25 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         wB[s][r] = ldB(s, r);
                     }
                 }
-                auto tap = [&](const int k) {
-                    float2 cf[R], cb[R];
+                // coefficients of taps k, k+1 (k even) for every x of the strip
+                auto fetch2 = [&](const int k, float2 (&f0)[R], float2 (&f1)[R], float2 (&b0)[R],
+                                  float2 (&b1)[R]) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        cf[r] = TF[qo[r] + k];
-                        cb[r] = CONJ ? make_float2(cf[r].x, -cf[r].y) : TB[qo[r] + k];
+                        const float4 v = *reinterpret_cast<const float4 *>(TF + qo[r] + k);
+                        f0[r] = make_float2(v.x, v.y);
+                        f1[r] = make_float2(v.z, v.w);
+                        if (CONJ) {
+                            b0[r] = make_float2(v.x, -v.y);
+                            b1[r] = make_float2(v.z, -v.w);
+                        } else {
+                            const float4 u = *reinterpret_cast<const float4 *>(TB + qo[r] + k);
+                            b0[r] = make_float2(u.x, u.y);
+                            b1[r] = make_float2(u.z, u.w);
+                        }
                     }
+                };
+                auto tap = [&](const int k, const float2 (&cf)[R], const float2 (&cb)[R]) {
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -209,26 +224,62 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         wF[s][R - 1] = ldF(s, R + k);  // the slot past the last tap is padding
                         wB[s][R - 1] = ldB(s, R + k);
                     }
-                    // wide windows: keep the next tap's R coefficient loads from
-                    // being hoisted over this one (bounds live registers)
+                };
+                auto tap2 = [&](const int k, const bool second) {
+                    float2 f0[R], f1[R], b0[R], b1[R];
+                    fetch2(k, f0, f1, b0, b1);
+                    tap(k, f0, b0);
+                    if (second)
+                        tap(k + 1, f1, b1);
+                    // wide windows: keep the next pair's loads from being hoisted
+                    // over this one (bounds live registers)
                     if constexpr (R >= 8)
                         __builtin_amdgcn_sched_barrier(0);
                 };
-                const int nch = L / R;
+                auto tap1 = [&](const int k) {
+                    float2 f0[R], b0[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        f0[r] = TF[qo[r] + k];
+                        b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : TB[qo[r] + k];
+                    }
+                    tap(k, f0, b0);
+                    if constexpr (R >= 8)
+                        __builtin_amdgcn_sched_barrier(0);
+                };
+                const int nch = L / R;  // R is even: chunks start at even taps
 #pragma unroll 1
                 for (int c = 0; c < nch; ++c) {
+                    if constexpr (PAIRED) {
 #pragma unroll
-                    for (int r2 = 0; r2 < R; ++r2)
-                        tap(c * R + r2);
+                        for (int r2 = 0; r2 < R; r2 += 2)
+                            tap2(c * R + r2, true);
+                    } else {
+#pragma unroll
+                        for (int r2 = 0; r2 < R; ++r2)
+                            tap1(c * R + r2);
+                    }
                 }
-                if (L_ > 0) {
+                if constexpr (PAIRED) {
+                    if (L_ > 0) {
 #pragma unroll
-                    for (int k = (L_ / R) * R; k < L_; ++k)
-                        tap(k);
-                } else {
+                        for (int k = (L_ / R) * R; k < L_; k += 2)
+                            tap2(k, k + 1 < L_);
+                    } else {
 #pragma unroll 1
-                    for (int k = nch * R; k < L; ++k)
-                        tap(k);
+                        for (int k = nch * R; k < L; k += 2)
+                            tap2(k, k + 1 < L);
+                    }
+                } else {
+                    if (L_ > 0) {
+#pragma unroll
+                        for (int k = (L_ / R) * R; k < L_; ++k)
+                            tap1(k);
+                    } else {
+#pragma unroll 1
+                        for (int k = nch * R; k < L; ++k)
+                            tap1(k);
+                    }
                 }
             } else {
                 // Tap-symmetric operator (T[k] == T[L-1-k]):
@@ -246,13 +297,24 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         rBw[s][r] = ldB(s, r + 2 * M);
                     }
                 }
-                auto pair = [&](const int k) {
-                    float2 cf[R], cb[R];
+                auto fetch2 = [&](const int k, float2 (&f0)[R], float2 (&f1)[R], float2 (&b0)[R],
+                                  float2 (&b1)[R]) {
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        cf[r] = TF[qo[r] + k];
-                        cb[r] = CONJ ? make_float2(cf[r].x, -cf[r].y) : TB[qo[r] + k];
+                        const float4 v = *reinterpret_cast<const float4 *>(TF + qo[r] + k);
+                        f0[r] = make_float2(v.x, v.y);
+                        f1[r] = make_float2(v.z, v.w);
+                        if (CONJ) {
+                            b0[r] = make_float2(v.x, -v.y);
+                            b1[r] = make_float2(v.z, -v.w);
+                        } else {
+                            const float4 u = *reinterpret_cast<const float4 *>(TB + qo[r] + k);
+                            b0[r] = make_float2(u.x, u.y);
+                            b1[r] = make_float2(u.z, u.w);
+                        }
                     }
+                };
+                auto pair = [&](const int k, const float2 (&cf)[R], const float2 (&cb)[R]) {
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -280,24 +342,61 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         rFw[s][0] = ldF(s, 2 * M - 1 - k);
                         rBw[s][0] = ldB(s, 2 * M - 1 - k);
                     }
+                };
+                auto pair2 = [&](const int k, const bool second) {
+                    float2 f0[R], f1[R], b0[R], b1[R];
+                    fetch2(k, f0, f1, b0, b1);
+                    pair(k, f0, b0);
+                    if (second)
+                        pair(k + 1, f1, b1);
+                    if constexpr (R >= 8)
+                        __builtin_amdgcn_sched_barrier(0);
+                };
+                auto pair1 = [&](const int k) {
+                    float2 f0[R], b0[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        f0[r] = TF[qo[r] + k];
+                        b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : TB[qo[r] + k];
+                    }
+                    pair(k, f0, b0);
                     if constexpr (R >= 8)
                         __builtin_amdgcn_sched_barrier(0);
                 };
                 const int nch = M / R;
 #pragma unroll 1
                 for (int c = 0; c < nch; ++c) {
+                    if constexpr (PAIRED) {
 #pragma unroll
-                    for (int r2 = 0; r2 < R; ++r2)
-                        pair(c * R + r2);
+                        for (int r2 = 0; r2 < R; r2 += 2)
+                            pair2(c * R + r2, true);
+                    } else {
+#pragma unroll
+                        for (int r2 = 0; r2 < R; ++r2)
+                            pair1(c * R + r2);
+                    }
                 }
-                if (L_ > 0) {
+                constexpr int M_ = L_ > 0 ? (L_ - 1) / 2 : 0;
+                if constexpr (PAIRED) {
+                    if (L_ > 0) {
 #pragma unroll
-                    for (int k = (((L_ - 1) / 2) / R) * R; k < (L_ - 1) / 2; ++k)
-                        pair(k);
-                } else {
+                        for (int k = (M_ / R) * R; k < M_; k += 2)
+                            pair2(k, k + 1 < M_);
+                    } else {
 #pragma unroll 1
-                    for (int k = nch * R; k < M; ++k)
-                        pair(k);
+                        for (int k = nch * R; k < M; k += 2)
+                            pair2(k, k + 1 < M);
+                    }
+                } else {
+                    if (L_ > 0) {
+#pragma unroll
+                        for (int k = (M_ / R) * R; k < M_; ++k)
+                            pair1(k);
+                    } else {
+#pragma unroll 1
+                        for (int k = nch * R; k < M; ++k)
+                            pair1(k);
+                    }
                 }
                 float2 cf[R], cb[R];
 #pragma unroll
@@ -568,6 +667,19 @@ __global__ void expand_kernel(const float *__restrict__ slow, int slow_ld,
     }
 }
 
+// out[q][0..L] = in[q][0..L-1], 0: rows padded to an even length
+__global__ void table_pad_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int nk,
+                                 int L)
+{
+    const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const int Lp = L + 1;
+    if (e >= (long long)nk * Lp)
+        return;
+    const long long q = e / Lp;
+    const int k = (int)(e - q * Lp);
+    out[e] = k < L ? in[q * L + k] : make_float2(0.f, 0.f);
+}
+
 __global__ void fill_f32_kernel(float *p, long long n, float v)
 {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -584,10 +696,10 @@ hipError_t ensure_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int L_, int R, int S, bool CONJ, bool SYM, int C, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
 hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
 {
-    auto k = fused_kernel<L_, R, S, CONJ, SYM, C, MAXT>;
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT>;
     hipError_t e = ensure_lds(k, lds);
     if (e != hipSuccess)
         return e;
@@ -600,11 +712,18 @@ hipError_t launch_fused_RS(const FusedShape &sh, int n_items, size_t lds, hipStr
                            const FusedArgs &a)
 {
     const int b = sh.block;
-    if (sh.sym)
-        return sh.conj ? launch_fused_t<L_, R, S, true, true, C, MAXT>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, R, S, false, true, C, MAXT>(n_items, b, lds, st, a);
-    return sh.conj ? launch_fused_t<L_, R, S, true, false, C, MAXT>(n_items, b, lds, st, a)
-                   : launch_fused_t<L_, R, S, false, false, C, MAXT>(n_items, b, lds, st, a);
+#define WLSQ_FUSED_P(P)                                                                          \
+    if (sh.sym)                                                                                  \
+        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT>(n_items, b, lds, st, a) \
+                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT>(n_items, b, lds, st, a); \
+    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT>(n_items, b, lds, st, a)    \
+                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT>(n_items, b, lds, st, a);
+    if (sh.paired) {
+        WLSQ_FUSED_P(true)
+    } else {
+        WLSQ_FUSED_P(false)
+    }
+#undef WLSQ_FUSED_P
 }
 
 template <int L_>
@@ -764,6 +883,16 @@ hipError_t launch_expand(const float *slow, int slow_ld, const float *omega, flo
     dim3 grid((nx + 255) / 256, nf, nextrap);
     hipLaunchKernelGGL(expand_kernel, grid, block, 0, st, slow, slow_ld, omega, kappa0, inv_dk, nk,
                        table, out, nf, nx, L, layout);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStream_t st)
+{
+    const long long n = (long long)nk * (L + 1);
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(table_pad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out,
+                       nk, L);
     return hipGetLastError();
 }
 

@@ -158,7 +158,7 @@ class GpuLauncher:
         specs = [device.ShotSpec(f_off=o, b_off=o, img_off=0, slow_off=job.x0, ld=job.nx,
                                  img_ld=job.nx, slow_ld=p.model.nx, nx=job.nx, nz=job.nz,
                                  nf=job.nf, omega_off=0) for o in offs]
-        tun = batch.default_tuning(job.nx, job.nshots > 1, sv.conj, sv.sym)
+        tun = batch.default_tuning(job.nx, job.nshots > 1, sv.conj, sv.sym, job.nshots)
         plan = device.Plan(specs, t.M, tun)
         plan.launch(sv.slowness, omega, t.kappa0, t.dkappa, t.nk, sv.tf, sv.tb, F, B, image,
                     stream=stream)

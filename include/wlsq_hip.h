@@ -124,6 +124,8 @@ typedef struct {
                              with symmetric taps gives this); taps are then
                              paired: (old[x+k-M] + old[x+M-k]) * T[k], which
                              changes the rounding order, not the operator */
+    int tap_loads;        /* 0 = auto, 1 = one tap per coefficient load, 2 = two
+                             taps per 16-byte load */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
@@ -184,7 +186,9 @@ int wlsq_plan_create(wlsq_plan *out, const wlsq_shot *shots, int nshots, int M,
  * shots per group; any pointer may be NULL */
 int wlsq_plan_info(wlsq_plan plan, long long *points, int *n_items, int *block, int *lds_bytes,
                    int *x_per_thread, int *shots_per_group);
-/* Queue the plan's single launch on `stream`; adds into `image`. */
+/* Queue the plan's launch on `stream` (a row-padded copy of the tables into
+ * the plan's storage, then the fused kernel); adds into `image`. Launches of
+ * one plan on different streams must not overlap. */
 int wlsq_plan_launch(wlsq_plan plan, const float *slowness, const float *omega, float kappa0,
                      float dkappa, int nk, const wlsq_c32 *table_forw,
                      const wlsq_c32 *table_back, wlsq_c32 *forw, wlsq_c32 *back, float *image,

@@ -188,6 +188,8 @@ def test_ktable_fused_matches_oracle(shape):
     dict(x_per_thread=8), dict(omegas_per_item=1), dict(omegas_per_item=3, block=512),
     dict(tables_conjugate=1), dict(image_mode=1), dict(image_mode=2),
     dict(image_mode=2, shots_per_group=4), dict(x_per_thread=2, shots_per_group=4),
+    dict(tap_loads=2), dict(tap_loads=2, shots_per_group=1), dict(tap_loads=1, shots_per_group=1),
+    dict(tap_loads=2, x_per_thread=8),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
@@ -223,8 +225,10 @@ def test_ktable_device_tunings_agree(tuning):
     dict(shots_per_group=1), dict(shots_per_group=2), dict(shots_per_group=4),
     dict(x_per_thread=8), dict(omegas_per_item=3, block=512), dict(tables_conjugate=1),
     dict(x_per_thread=2, shots_per_group=2), dict(x_per_thread=2, shots_per_group=4),
+    dict(shots_per_group=1, tap_loads=2), dict(shots_per_group=2, tap_loads=2),
+    dict(x_per_thread=8, tap_loads=2), dict(x_per_thread=2, shots_per_group=2, tap_loads=2),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
-@pytest.mark.parametrize("M", [15, 4, 0, 25])
+@pytest.mark.parametrize("M", [15, 4, 0, 25, 1])
 def test_tap_paired_variant_matches_oracle(tuning, M):
     """Tap-symmetric tables (the WLSQ design gives them): the paired-tap
     kernel computes the same operator in a different rounding order."""
