@@ -135,6 +135,39 @@ def bench_c3(args, log):
     return res
 
 
+def bench_c5(args, log):
+    """Survey-scale shots (2000 x 8192 at 6.25 m, nf 1024 over 2-80 Hz, 51 taps
+    x 8192 kappa samples): --c5-shots full-aperture shots in one launch."""
+    import torch
+
+    from imaging2d_wlsq_amd import batch, synth
+
+    t0 = time.perf_counter()
+    p = synth.config_c5()
+    sv = batch.Survey(p)
+    shots = p.shots[:: max(1, len(p.shots) // args.c5_shots)][:args.c5_shots]
+    b = sv.batch(shots)
+    setup = time.perf_counter() - t0
+    b.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    pts_s = b.points / (ms / 1e3)
+    res = {"workload": p.name, "shots_measured": len(shots), "shots_in_survey": len(p.shots),
+           "points_per_launch": b.points, "value": pts_s, "unit": "points/s", "ms_per_launch": ms,
+           "tuning": b.tuning, "plan": b.plan.info(), "setup_s": setup,
+           "achieved_tflops": flops_per_point(p.tables.L) * b.points / (ms / 1e3) / 1e12,
+           "survey_points": p.points, "survey_estimated_s": p.points / pts_s}
+    log(f"c5: {len(shots)} shots {ms:.0f} ms, {pts_s:.3e} points/s, {res['achieved_tflops']:.1f} TFLOP/s, "
+        f"full survey ~{res['survey_estimated_s']:.0f} s")
+    b.close()
+    return res
+
+
 def bench_serving(args, log):
     """Continuous admission of jobs of mixed size against 4 resident models.
 
@@ -193,6 +226,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=200,
                     help="jobs of the serving run (0: skip; the full configuration is 2000)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--c5-shots", type=int, default=8,
+                    help="shots of the 2000 x 8192 survey measured (0: skip; the survey has 1000)")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="arrival rate as a multiple of the measured saturated service rate")
     ap.add_argument("--tune-verbose", action="store_true")
@@ -313,7 +348,7 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "fp32 VALU (no MFMA: per-point gather-convolution)",
                      "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-                     "traffic": read_profile_traffic("fused_kernel_c2_traffic.txt"),
+                     "traffic": read_profile_traffic("fused_kernel_c2_512x1024_nf256_16shots.txt"),
                      "kernel": "fused_kernel (L=31)", "kernel_ms": k_avg,
                      "flops_per_point": flops_per_point(L)},
     }
@@ -364,7 +399,7 @@ def main():
         out["image_rel_l2_batched_vs_plain"] = err
         out["roofline_plain"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                                 "traffic": read_profile_traffic("plain_step_c2_traffic.txt"),
+                                 "traffic": read_profile_traffic("plain_step_kernel_c2_512x1024_nf256_16shots.txt"),
                                  "kernel": "plain_step_kernel<1,31>",
                                  "bytes_per_point": plain_bytes_per_point(L),
                                  "note": "per-launch time = step time / (shots*depths); includes "
@@ -374,6 +409,9 @@ def main():
 
     if args.c3_steps > 0:
         out["c3_batched"] = bench_c3(args, log)
+        torch.cuda.empty_cache()
+    if args.c5_shots > 0:
+        out["c5_survey"] = bench_c5(args, log)
         torch.cuda.empty_cache()
     if args.serve_jobs > 0:
         out["serving"] = bench_serving(args, log)

@@ -130,3 +130,28 @@ def test_each_job_same_alone_and_in_the_mix():
         a, m = alone[j.job_id], st.images[j.job_id]
         assert rel(m, a) <= 1e-6, (j.job_id, rel(m, a))
     launcher.close()
+
+
+def test_survey_scale_shot_cropped_matches_oracle():
+    """A shot of the 2000 x 8192 survey model (51 taps, 8192 kappa samples),
+    cropped to 40 depth steps and 12 frequencies: the two-strips-per-thread
+    shape for 8192-wide apertures against the CPU oracle."""
+    import torch
+    from imaging2d_wlsq_amd import batch, synth
+    p = synth.config_c5(nshots=4)
+    s0 = p.shots[1]
+    freqs = s0.freqs[::85][:12]
+    shot = synth.ShotGeom(0, p.model.nx, 40, s0.xs, freqs, s0.seed)
+    sv = batch.Survey(p)
+    b = sv.batch([shot])
+    print("c5 crop", b.tuning, b.plan.info(), flush=True)
+    img = b.run().clone()
+    torch.cuda.synchronize()
+    m, t = p.model, p.tables
+    F = b.F.cpu().numpy().reshape(1, shot.nf, shot.nx).copy()
+    B = b.B.cpu().numpy().reshape(1, shot.nf, shot.nx).copy()
+    io = np.zeros((1, 40, m.nx), np.float32)
+    oracle.extrapolate_ktable(1, 40, 40, m.nx, shot.nf, shot.nf, t.M, np.ascontiguousarray(m.slowness[:40]),
+                              shot.omega, t.kappa0, t.dkappa, t.nk, t.forw, t.back, F, B, io)
+    assert rel(img.cpu().numpy()[:40], io[0]) <= 1e-5
+    b.close()
