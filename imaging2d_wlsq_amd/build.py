@@ -27,7 +27,11 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libwlsq_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["wlsq_kernels.hip", "wlsq_capi.hip"]
+# (source, object, extra flags): the fused kernel is built once per taps
+# (25, 31, 51; 0 = any, runtime loop bounds) and gather width, in parallel
+SOURCES = [("wlsq_kernels.hip", "wlsq_kernels.o", []), ("wlsq_capi.hip", "wlsq_capi.o", [])] + [
+    ("wlsq_fused.hip", f"wlsq_fused_L{l}_P{p}.o", [f"-DWLSQ_L={l}", f"-DWLSQ_PAIRED={p}"])
+    for l in (25, 31, 51, 0) for p in (0, 1)]
 
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
@@ -49,11 +53,12 @@ def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     cmds = []
-    for src in SOURCES:
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    for src, o, extra in SOURCES:
+        obj = os.path.join(BUILD, o)
         objs.append(obj)
-        cmds.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        cmds.append([HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         for r, c in zip(ex.map(_run, cmds), cmds):
             if verbose:
                 print(" ".join(c))

@@ -224,6 +224,7 @@ struct PlanImpl {
     FusedShape shape{4, 1, 1, 256, false, false, false};
     size_t lds = 0;
     int img_lds = 1;
+    int strip_gathers = 1;
     long long points = 0;
     std::vector<Group> groups;
     std::vector<Item> items;
@@ -295,10 +296,17 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         P->points += (long long)s.nx * s.nz * s.nf;
     }
 
-    // x per thread and shots per group
+    // longest run of consecutive same-shaped shots (they can share a group)
+    int best_run = 1;
+    for (int i = 1, run = 1; i < nshots; ++i) {
+        run = same_shape(shots[i], shots[i - 1]) ? run + 1 : 1;
+        best_run = std::max(best_run, run);
+    }
+    // x per thread and shots per group; the defaults follow an autotuner
+    // sweep on MI355X over apertures 512-8192 and 1-16 shots per group
     int R = t.x_per_thread;
     if (R == 0)
-        R = (max_nx + 3) / 4 <= 512 ? 4 : 8;
+        R = max_nx > 2048 ? 8 : (best_run >= 2 && max_nx <= 1024 && t.shots_per_group != 1) ? 2 : 4;
     if (R != 2 && R != 4 && R != 8)
         return fail(WLSQ_EINVAL, "x_per_thread must be 2, 4 or 8 (got %d)", R);
     const int max_block = 512;
@@ -309,14 +317,8 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
                     max_strips, R, (R == 8 ? 2 : 1) * max_block);
     int S = t.shots_per_group;
-    if (S == 0) {
-        int run = 1, best = 1;
-        for (int i = 1; i < nshots; ++i) {
-            run = same_shape(shots[i], shots[i - 1]) ? run + 1 : 1;
-            best = std::max(best, run);
-        }
-        S = R == 8 ? 1 : R == 2 ? (best >= 4 ? 4 : 2) : (best >= 4 ? 4 : best >= 2 ? 2 : 1);
-    }
+    if (S == 0)
+        S = R == 8 ? 1 : R == 2 ? (best_run >= 8 ? 4 : 2) : (best_run >= 2 ? 2 : 1);
     if (S != 1 && S != 2 && S != 4)
         return fail(WLSQ_EINVAL, "shots_per_group must be 1, 2 or 4 (got %d)", S);
     if (R == 8 && S != 1)
@@ -327,8 +329,8 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     int block = t.block;
     if (block == 0) {
         block = (((max_strips + C - 1) / C + 63) / 64) * 64;
-        if (block < 256)
-            block = 256;
+        if (block < 256 || R == 4)
+            block = std::max(block, R == 4 ? 512 : 256);
     }
     if (block < 64 || block > max_block || block % 64 || block * C < max_strips)
         return fail(WLSQ_EINVAL, "block=%d invalid for %d strips (multiple of 64, <= %d)", block,
@@ -406,6 +408,8 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         int W = block * C / strips;
         if (t.omegas_per_item > 0)
             W = std::min(W, t.omegas_per_item);
+        else if (best_run < 2 && R == 4)
+            W = 1;  // unshared shots: one frequency per task spreads the work widest
         W = std::min(W, s.nf);
         while (W > 1 && fused_lds_bytes(s.nx, M, R, S, W, img_rows) > budget)
             --W;
@@ -443,6 +447,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
     P->lds = lds_need;
     P->img_lds = img_lds;
+    if (t.strip_gathers < 0 || t.strip_gathers > 2)
+        return fail(WLSQ_EINVAL, "strip_gathers must be 0, 1 or 2 (got %d)", t.strip_gathers);
+    P->strip_gathers = t.strip_gathers == 2 ? 0 : 1;
     if (!P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
@@ -489,6 +496,7 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     a.L = P.L;
     a.write_back = write_back;
     a.img_lds = P.img_lds;
+    a.strip_gathers = P.strip_gathers;
     HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
     return WLSQ_OK;
 }
@@ -537,7 +545,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
                                t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
-                               (long long)(intptr_t)st};  // a plan's table copies are per stream
+                               t.strip_gathers, (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);

@@ -1,0 +1,58 @@
+// Device helpers shared by the kernel translation units of libwlsq_hip.
+#pragma once
+#include "wlsq_internal.h"
+
+namespace wlsq {
+namespace {
+
+// acc + a*b (complex); the product's parts are formed with fused multiply-adds.
+__device__ __forceinline__ float2 cmac(float2 acc, float2 a, float2 b)
+{
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+
+// Re(a * conj(b))
+__device__ __forceinline__ float re_mul_conj(float2 a, float2 b)
+{
+    return fmaf(a.y, b.y, a.x * b.x);
+}
+
+// Nearest kappa sample to omega*s on kappa_q = kappa0 + q/inv_dk, ties to the
+// lower q (numpy argmin in py_src/operatorsPreparation/util.py:5-8). One
+// rounding per operation and no contraction, so the host-side expansion and
+// the CPU restatement compute the same q bit for bit.
+__device__ __forceinline__ int kappa_index(float omega, float s, float kappa0, float inv_dk, int nk)
+{
+#pragma clang fp contract(off)
+    float t = omega * s;
+    t = t - kappa0;
+    float u = t * inv_dk;
+    const float hi = (float)(nk - 1);
+    u = (u > 0.0f) ? u : 0.0f;
+    u = (u > hi) ? hi : u;
+    return (int)ceilf(u - 0.5f);
+}
+
+// Position p of a padded row lives at sub-row (p mod R), slot p / R: lanes
+// that read or write p = R*lane + c for a fixed c touch consecutive slots.
+template <int R>
+__device__ __forceinline__ int swz(int p, int Q)
+{
+    return (p & (R - 1)) * Q + p / R;
+}
+
+template <typename K>
+hipError_t ensure_lds(K kernel, size_t bytes)
+{
+    if (bytes <= 64 * 1024)
+        return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace
+}  // namespace wlsq

@@ -1,0 +1,579 @@
+// gfx950 fused extrapolate-and-image kernel (compact operator tables).
+//
+// A workgroup owns complete rows (S shots x W frequencies) of one shot group
+// in LDS for ALL depth steps, gathers each operator from a compact (nk x L)
+// table by the kappa lookup and adds the image with atomics. Semantics as in
+// py_src/wavefieldPropagation/extrapolation.py:57-64 and
+// cpp_src/extrapolation_cpu.cpp:39-89 (unconjugated dot, taps in order).
+//
+// This file is compiled once per (WLSQ_L, WLSQ_PAIRED): taps 25, 31, 51 or 0
+// (any L, runtime loop bounds) and 8- or 16-byte coefficient gathers, so the
+// variants build in parallel. Each object defines launch_fused_L<L>_P<P>.
+#include "wlsq_device.h"
+
+#ifndef WLSQ_L
+#error "compile with -DWLSQ_L=<taps or 0> -DWLSQ_PAIRED=<0|1>"
+#endif
+
+namespace wlsq {
+namespace {
+
+// ------------------------------------------------------------------ fused kernel
+// LDS holds one 16-byte element (F, B) per position of each padded row, so a
+// window load is one ds_read_b128 for both fields. Position p of a row lives
+// at sub-row (p mod R), slot p / R (swz): lanes that read p = R*lane + c for a
+// fixed c touch consecutive 16-byte slots.
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
+__global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
+{
+    constexpr int LOG2R = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
+    static_assert((1 << LOG2R) == R, "R must be a power of two");
+    constexpr int M_ = L_ > 0 ? (L_ - 1) / 2 : 0;
+    extern __shared__ float4 lds4[];
+    const int tid = threadIdx.x;
+    const int NT = blockDim.x;
+    const Item it = a.items[blockIdx.x];
+    const Group *gp = a.groups + it.group;
+    const int nshots = gp->nshots;
+    const int nx = gp->nx;
+    const int nz = gp->nz;
+    const int ld = gp->ld;
+    const int L = L_ > 0 ? L_ : a.L;
+    const int M = L_ > 0 ? M_ : (L - 1) >> 1;
+    const int nstrips = (nx + R - 1) / R;
+    const int nxr = nstrips * R;
+    // padded row: M halo + nxr + M halo + 1 slot read by the final window slide
+    const int Q = (nxr + 2 * M + R) / R;
+    const int rowlen = Q * R;
+    const int W = it.nw;
+    const int nrows = S * W;
+    if (W * nstrips > C * NT)
+        return;  // the planner never builds such a task
+    float4 *rows = lds4;  // row (s * W + w): shot s, frequency j0 + w
+    float *imgrow = reinterpret_cast<float *>(lds4 + (size_t)nrows * rowlen);
+
+    // Initial rows: interior from the fields, zero halos and padding.
+    for (int e = tid; e < nrows * rowlen; e += NT) {
+        const int row = e / rowlen;
+        const int p = e - row * rowlen;
+        const int s = row / W;
+        const int w = row - s * W;
+        const int x = p - M;
+        float2 vf = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+        if (s < nshots && x >= 0 && x < nx) {
+            const long long r = (long long)(it.j0 + w) * ld + x;
+            vf = a.F[gp->f_off[s] + r];
+            vb = a.B[gp->b_off[s] + r];
+        }
+        rows[row * rowlen + swz<R>(p, Q)] = make_float4(vf.x, vf.y, vb.x, vb.y);
+    }
+    const bool shared_img = gp->img_shared != 0;
+    const int img_rows = shared_img ? 1 : nshots;  // image rows reduced in LDS
+    if (a.img_lds)
+        for (int e = tid; e < img_rows * nxr; e += NT)
+            imgrow[e] = 0.f;
+
+    long long ioff[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+        ioff[s] = gp->img_off[s];
+    const int img_ld = gp->img_ld;
+    const int slow_ld = gp->slow_ld;
+    const float *slow0 = a.slow + gp->slow_off;
+    // tables have rows of Lp = L + 1 entries (16-byte aligned): taps are
+    // gathered in pairs, one 16-byte load per pair and table
+    const float2 *__restrict__ TF = a.TF;
+    const float2 *__restrict__ TB = a.TB;
+    const int Lp = L + 1;
+
+    // Each thread owns C strips (task slots tid, tid + NT, ...): slot ->
+    // (frequency row w, strip of R consecutive x).
+    const size_t shot_rows = (size_t)W * rowlen;
+    bool actv[C];
+    int wv[C], x0v[C];
+    float omv[C];
+#pragma unroll
+    for (int cs = 0; cs < C; ++cs) {
+        const int slot = tid + cs * NT;
+        actv[cs] = slot < W * nstrips;
+        wv[cs] = actv[cs] ? slot / nstrips : 0;
+        x0v[cs] = actv[cs] ? (slot - wv[cs] * nstrips) * R : 0;
+        omv[cs] = actv[cs] ? a.omega[gp->omega_off + it.j0 + wv[cs]] : 0.f;
+    }
+    __syncthreads();
+
+    float2 accF_[C][S][R], accB_[C][S][R];
+    for (int l = 0; l < nz; ++l) {
+        // one depth step of strip cs (a compile-time index: the accumulators
+        // stay in registers)
+        auto strip_step = [&](auto cs_c) {
+        constexpr int cs = decltype(cs_c)::value;
+        if (actv[cs]) {
+            float2 (&accF)[S][R] = accF_[cs];
+            float2 (&accB)[S][R] = accB_[cs];
+            const int x0 = x0v[cs];
+            const float om = omv[cs];
+            const float *srow = slow0 + (long long)l * slow_ld;
+            int qo[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int xr = min(x0 + r, nx - 1);
+                qo[r] = kappa_index(om, srow[xr], a.kappa0, a.inv_dk, a.nk) * Lp;
+            }
+            // Kappa rows are piecewise constant along x. When every active lane
+            // of the wavefront has one row over its strip (mode 1), or one
+            // change (mode 2: a prefix on row qo[0], the rest on qo[R-1]), the
+            // strip's coefficients come from one or two gathers per tap instead
+            // of R (mode 0). Every mode uses the same coefficients.
+            bool sel[R];
+            int changes = 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sel[r] = qo[r] == qo[0];
+                if (r > 0)
+                    changes += qo[r] != qo[r - 1] ? 1 : 0;
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    accF[s][r] = make_float2(0.f, 0.f);
+                    accB[s][r] = make_float2(0.f, 0.f);
+                }
+            }
+            // x0 is a multiple of R, so p = x0 + j is at sub-row (j mod R),
+            // slot x0 / R + j / R: one base per (shot, sub-row), and with a
+            // fixed tap count every load is that base plus an immediate.
+            const int xs = x0 / R;
+            const float4 *bs[S][R];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    bs[s][r] = rows + (size_t)s * shot_rows + (size_t)wv[cs] * rowlen + r * Q + xs;
+            }
+            auto ldw = [&](int s, int j) { return bs[s][j & (R - 1)][j >> LOG2R]; };
+            auto step = [&](auto mode_c) {
+                constexpr int MODE = decltype(mode_c)::value;
+                auto ld4 = [](const float2 *T, int o) { return *reinterpret_cast<const float4 *>(T + o); };
+                // coefficients of taps k, k+1 (k even) for every x of the strip
+                auto fetch2 = [&](const int k, float2 (&f0)[R], float2 (&f1)[R], float2 (&b0)[R],
+                                  float2 (&b1)[R]) {
+                    float4 va{}, vb{}, ua{}, ub{};
+                    if constexpr (MODE != 0) {
+                        va = ld4(TF, qo[0] + k);
+                        if constexpr (!CONJ)
+                            ua = ld4(TB, qo[0] + k);
+                        if constexpr (MODE == 2) {
+                            vb = ld4(TF, qo[R - 1] + k);
+                            if constexpr (!CONJ)
+                                ub = ld4(TB, qo[R - 1] + k);
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        float4 v, u{};
+                        if constexpr (MODE == 0) {
+                            v = ld4(TF, qo[r] + k);
+                            if constexpr (!CONJ)
+                                u = ld4(TB, qo[r] + k);
+                        } else if constexpr (MODE == 1) {
+                            v = va;
+                            u = ua;
+                        } else {
+                            v = sel[r] ? va : vb;
+                            if constexpr (!CONJ)
+                                u = sel[r] ? ua : ub;
+                        }
+                        f0[r] = make_float2(v.x, v.y);
+                        f1[r] = make_float2(v.z, v.w);
+                        if (CONJ) {
+                            b0[r] = make_float2(v.x, -v.y);
+                            b1[r] = make_float2(v.z, -v.w);
+                        } else {
+                            b0[r] = make_float2(u.x, u.y);
+                            b1[r] = make_float2(u.z, u.w);
+                        }
+                    }
+                };
+                // coefficient of tap k for every x of the strip
+                auto fetch1 = [&](const int k, float2 (&f0)[R], float2 (&b0)[R]) {
+                    float2 va{}, vb{}, ua{}, ub{};
+                    if constexpr (MODE != 0) {
+                        va = TF[qo[0] + k];
+                        if constexpr (!CONJ)
+                            ua = TB[qo[0] + k];
+                        if constexpr (MODE == 2) {
+                            vb = TF[qo[R - 1] + k];
+                            if constexpr (!CONJ)
+                                ub = TB[qo[R - 1] + k];
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if constexpr (MODE == 0) {
+                            f0[r] = TF[qo[r] + k];
+                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : TB[qo[r] + k];
+                        } else {
+                            f0[r] = (MODE == 1 || sel[r]) ? va : vb;
+                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : ((MODE == 1 || sel[r]) ? ua : ub);
+                        }
+                    }
+                };
+                // wide windows: keep the next taps' loads from being hoisted over
+                // this one (bounds live registers)
+                auto fence = [] {
+                    if constexpr (R >= 8)
+                        __builtin_amdgcn_sched_barrier(0);
+                };
+                if constexpr (!SYM) {
+                    // new[x] = sum_k T[k] old[x + k - M]: a window of R inputs slides
+                    // right by one per tap. Taps go in chunks of R so the window
+                    // registers are back in place after each chunk.
+                    float4 wv4[S][R];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            wv4[s][r] = ldw(s, r);
+                    }
+                    auto tap = [&](const int k, const float2 (&cf)[R], const float2 (&cb)[R]) {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                accF[s][r] = cmac(accF[s][r], cf[r], make_float2(wv4[s][r].x, wv4[s][r].y));
+                                accB[s][r] = cmac(accB[s][r], cb[r], make_float2(wv4[s][r].z, wv4[s][r].w));
+                            }
+                        }
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+#pragma unroll
+                            for (int r = 0; r + 1 < R; ++r)
+                                wv4[s][r] = wv4[s][r + 1];
+                            wv4[s][R - 1] = ldw(s, R + k);  // the slot past the last tap is padding
+                        }
+                    };
+                    auto tap2 = [&](const int k, const bool second) {
+                        float2 f0[R], f1[R], b0[R], b1[R];
+                        fetch2(k, f0, f1, b0, b1);
+                        tap(k, f0, b0);
+                        if (second)
+                            tap(k + 1, f1, b1);
+                        fence();
+                    };
+                    auto tap1 = [&](const int k) {
+                        float2 f0[R], b0[R];
+                        fetch1(k, f0, b0);
+                        tap(k, f0, b0);
+                        fence();
+                    };
+                    auto chunk = [&](const int c) {
+                        if constexpr (PAIRED) {
+#pragma unroll
+                            for (int r2 = 0; r2 < R; r2 += 2)
+                                tap2(c * R + r2, true);
+                        } else {
+#pragma unroll
+                            for (int r2 = 0; r2 < R; ++r2)
+                                tap1(c * R + r2);
+                        }
+                    };
+                    if constexpr (L_ > 0) {
+                        constexpr int NCH = L_ / R;  // R is even: chunks start at even taps
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c)
+                            chunk(c);
+                        if constexpr (PAIRED) {
+#pragma unroll
+                            for (int k = NCH * R; k < L_; k += 2)
+                                tap2(k, k + 1 < L_);
+                        } else {
+#pragma unroll
+                            for (int k = NCH * R; k < L_; ++k)
+                                tap1(k);
+                        }
+                    } else {
+                        const int nch = L / R;
+#pragma unroll 1
+                        for (int c = 0; c < nch; ++c)
+                            chunk(c);
+                        if constexpr (PAIRED) {
+#pragma unroll 1
+                            for (int k = nch * R; k < L; k += 2)
+                                tap2(k, k + 1 < L);
+                        } else {
+#pragma unroll 1
+                            for (int k = nch * R; k < L; ++k)
+                                tap1(k);
+                        }
+                    }
+                } else {
+                    // Tap-symmetric operator (T[k] == T[L-1-k]):
+                    //   new[x] = T[M] old[x] + sum_{k<M} T[k] (old[x+k-M] + old[x+M-k]).
+                    // A left window slides right and a right window slides left;
+                    // after the M pairs the left window holds old[x] for the centre.
+                    float4 lw[S][R], rw[S][R];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            lw[s][r] = ldw(s, r);
+                            rw[s][r] = ldw(s, r + 2 * M);
+                        }
+                    }
+                    auto pair = [&](const int k, const float2 (&cf)[R], const float2 (&cb)[R]) {
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                const float4 u = lw[s][r], v = rw[s][r];
+                                accF[s][r] = cmac(accF[s][r], cf[r], make_float2(u.x + v.x, u.y + v.y));
+                                accB[s][r] = cmac(accB[s][r], cb[r], make_float2(u.z + v.z, u.w + v.w));
+                            }
+                        }
+#pragma unroll
+                        for (int s = 0; s < S; ++s) {
+#pragma unroll
+                            for (int r = 0; r + 1 < R; ++r)
+                                lw[s][r] = lw[s][r + 1];
+                            lw[s][R - 1] = ldw(s, R + k);
+#pragma unroll
+                            for (int r = R - 1; r > 0; --r)
+                                rw[s][r] = rw[s][r - 1];
+                            rw[s][0] = ldw(s, 2 * M - 1 - k);
+                        }
+                    };
+                    auto pair2 = [&](const int k, const bool second) {
+                        float2 f0[R], f1[R], b0[R], b1[R];
+                        fetch2(k, f0, f1, b0, b1);
+                        pair(k, f0, b0);
+                        if (second)
+                            pair(k + 1, f1, b1);
+                        fence();
+                    };
+                    auto pair1 = [&](const int k) {
+                        float2 f0[R], b0[R];
+                        fetch1(k, f0, b0);
+                        pair(k, f0, b0);
+                        fence();
+                    };
+                    auto chunk = [&](const int c) {
+                        if constexpr (PAIRED) {
+#pragma unroll
+                            for (int r2 = 0; r2 < R; r2 += 2)
+                                pair2(c * R + r2, true);
+                        } else {
+#pragma unroll
+                            for (int r2 = 0; r2 < R; ++r2)
+                                pair1(c * R + r2);
+                        }
+                    };
+                    if constexpr (L_ > 0) {
+                        constexpr int NCH = M_ / R;
+#pragma unroll
+                        for (int c = 0; c < NCH; ++c)
+                            chunk(c);
+                        if constexpr (PAIRED) {
+#pragma unroll
+                            for (int k = NCH * R; k < M_; k += 2)
+                                pair2(k, k + 1 < M_);
+                        } else {
+#pragma unroll
+                            for (int k = NCH * R; k < M_; ++k)
+                                pair1(k);
+                        }
+                    } else {
+                        const int nch = M / R;
+#pragma unroll 1
+                        for (int c = 0; c < nch; ++c)
+                            chunk(c);
+                        if constexpr (PAIRED) {
+#pragma unroll 1
+                            for (int k = nch * R; k < M; k += 2)
+                                pair2(k, k + 1 < M);
+                        } else {
+#pragma unroll 1
+                            for (int k = nch * R; k < M; ++k)
+                                pair1(k);
+                        }
+                    }
+                    float2 cf[R], cb[R];
+                    fetch1(M, cf, cb);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            accF[s][r] = cmac(accF[s][r], cf[r], make_float2(lw[s][r].x, lw[s][r].y));
+                            accB[s][r] = cmac(accB[s][r], cb[r], make_float2(lw[s][r].z, lw[s][r].w));
+                        }
+                    }
+                }
+            };
+            // The mode is uniform over the wavefront (a scalar branch). Groups of
+            // several shots already share each gather over the shots and are
+            // VALU-bound: they keep one gather per x and its register budget.
+            if constexpr (S == 1) {
+                if (a.strip_gathers && __ballot(changes != 0) == 0)
+                    step(std::integral_constant<int, 1>{});
+                else if (a.strip_gathers && __ballot(changes > 1) == 0)
+                    step(std::integral_constant<int, 2>{});
+                else
+                    step(std::integral_constant<int, 0>{});
+            } else {
+                (void)changes;
+                step(std::integral_constant<int, 0>{});
+            }
+            // imaging condition of this step. With img_lds the task's rows are
+            // summed in LDS (over its frequencies, and over its shots when they
+            // share an image) and flushed once per (depth, x) below.
+            if (a.img_lds) {
+                if (shared_img) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        float v = 0.f;
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                            if (s < nshots)
+                                v += re_mul_conj(accF[s][r], accB[s][r]);
+                        if (x0 + r < nx)
+                            atomicAdd(&imgrow[x0 + r], v);
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        if (s < nshots) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                if (x0 + r < nx)
+                                    atomicAdd(&imgrow[s * nxr + x0 + r],
+                                              re_mul_conj(accF[s][r], accB[s][r]));
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s < nshots) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            const int x = x0 + r;
+                            if (x < nx)
+                                unsafeAtomicAdd(a.img + ioff[s] + (long long)l * img_ld + x,
+                                                re_mul_conj(accF[s][r], accB[s][r]));
+                        }
+                    }
+                }
+            }
+        }
+        };
+        strip_step(std::integral_constant<int, 0>{});
+        if constexpr (C == 2)
+            strip_step(std::integral_constant<int, 1>{});
+        static_assert(C == 1 || C == 2, "one or two strips per thread");
+        __syncthreads();
+#pragma unroll
+        for (int cs = 0; cs < C; ++cs) {
+            if (!actv[cs])
+                continue;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int x = x0v[cs] + r;
+                    if (x < nx) {
+                        const float2 f = accF_[cs][s][r], b = accB_[cs][s][r];
+                        rows[s * shot_rows + (size_t)wv[cs] * rowlen + swz<R>(x + M, Q)] =
+                            make_float4(f.x, f.y, b.x, b.y);
+                    }
+                }
+            }
+        }
+        if (a.img_lds) {
+            // consecutive lanes flush consecutive x: one 32-byte sector of the
+            // image carries 8 atomic adds
+            for (int e = tid; e < img_rows * nx; e += NT) {
+                const int s = e / nx;
+                const int x = e - s * nx;
+                const float v = imgrow[s * nxr + x];
+                imgrow[s * nxr + x] = 0.f;
+                unsafeAtomicAdd(a.img + gp->img_off[s] + (long long)l * img_ld + x, v);
+            }
+        }
+        __syncthreads();
+    }
+
+    if (a.write_back) {
+        for (int e = tid; e < nrows * nx; e += NT) {
+            const int row = e / nx;
+            const int x = e - row * nx;
+            const int s = row / W;
+            const int ww = row - s * W;
+            if (s < nshots) {
+                const float4 v = rows[row * rowlen + swz<R>(x + M, Q)];
+                const long long r = (long long)(it.j0 + ww) * ld + x;
+                a.F[gp->f_off[s] + r] = make_float2(v.x, v.y);
+                a.B[gp->b_off[s] + r] = make_float2(v.z, v.w);
+            }
+        }
+    }
+}
+
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
+hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
+{
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT>;
+    hipError_t e = ensure_lds(k, lds);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k, dim3(n_items), dim3(block), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int L_, bool P, int R, int S, int C, int MAXT>
+hipError_t launch_fused_RS(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
+                           const FusedArgs &a)
+{
+    const int b = sh.block;
+    if (sh.sym)
+        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT>(n_items, b, lds, st, a)
+                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT>(n_items, b, lds, st, a);
+    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT>(n_items, b, lds, st, a)
+                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT>(n_items, b, lds, st, a);
+}
+
+template <int L_, bool P>
+hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
+                          const FusedArgs &a)
+{
+    if (sh.C == 1) {
+        if (sh.R == 4 && sh.S == 1)
+            return launch_fused_RS<L_, P, 4, 1, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 4 && sh.S == 2)
+            return launch_fused_RS<L_, P, 4, 2, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 4 && sh.S == 4)
+            return launch_fused_RS<L_, P, 4, 4, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 8 && sh.S == 1)
+            return launch_fused_RS<L_, P, 8, 1, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 2 && sh.S == 2)
+            return launch_fused_RS<L_, P, 2, 2, 1, 512>(sh, n_items, lds, st, a);
+        if (sh.R == 2 && sh.S == 4)
+            return launch_fused_RS<L_, P, 2, 4, 1, 512>(sh, n_items, lds, st, a);
+    } else if (sh.C == 2) {
+        if (sh.R == 8 && sh.S == 1)
+            return launch_fused_RS<L_, P, 8, 1, 2, 512>(sh, n_items, lds, st, a);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+#define WLSQ_FUSED_NAME2(l, p) launch_fused_L##l##_P##p
+#define WLSQ_FUSED_NAME(l, p) WLSQ_FUSED_NAME2(l, p)
+hipError_t WLSQ_FUSED_NAME(WLSQ_L, WLSQ_PAIRED)(const FusedShape &sh, int n_items, size_t lds,
+                                                hipStream_t st, const FusedArgs &a)
+{
+    return launch_fused_L<WLSQ_L, WLSQ_PAIRED != 0>(sh, n_items, lds, st, a);
+}
+
+}  // namespace wlsq

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "wlsq_hip.h"
 
@@ -55,6 +56,7 @@ struct FusedArgs {
     int L;
     int write_back;
     int img_lds;  // 1: reduce image rows in LDS, flush with lane-contiguous atomics
+    int strip_gathers;  // 1: one or two gathers per strip where the kappa rows allow it
 };
 
 // Launch shape of the fused kernel; chosen by the planner.
@@ -74,6 +76,20 @@ size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows);
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,
                         const FusedArgs &a);
+// One object of wlsq_fused.hip per taps (25, 31, 51, 0 = any) and gather width
+// (P1: two taps per 16-byte load).
+#define WLSQ_DECLARE_FUSED(l, p)                                                              \
+    hipError_t launch_fused_L##l##_P##p(const FusedShape &sh, int n_items, size_t lds,        \
+                                        hipStream_t st, const FusedArgs &a);
+WLSQ_DECLARE_FUSED(25, 0)
+WLSQ_DECLARE_FUSED(25, 1)
+WLSQ_DECLARE_FUSED(31, 0)
+WLSQ_DECLARE_FUSED(31, 1)
+WLSQ_DECLARE_FUSED(51, 0)
+WLSQ_DECLARE_FUSED(51, 1)
+WLSQ_DECLARE_FUSED(0, 0)
+WLSQ_DECLARE_FUSED(0, 1)
+#undef WLSQ_DECLARE_FUSED
 
 // Plain path kernels.
 hipError_t launch_transpose_tapmajor(const float2 *nat, float2 *tap, long long n_slices, int nx,

@@ -126,6 +126,11 @@ typedef struct {
                              changes the rounding order, not the operator */
     int tap_loads;        /* 0 = auto, 1 = one tap per coefficient load, 2 = two
                              taps per 16-byte load */
+    int strip_gathers;    /* 0 = auto (on), 1 = on: where every lane of a
+                             wavefront sees at most one change of kappa row
+                             over its x_per_thread positions, gather each tap
+                             once or twice per lane instead of once per x;
+                             2 = off. Results are bitwise the same. */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

@@ -262,6 +262,69 @@ def test_tap_paired_variant_matches_oracle(tuning, M):
     assert crel(outs[1][1], outs[0][1]) <= 1e-6
 
 
+def _blocky_slowness(rng, nextrap, nx, noisy_from):
+    """Rows of constant runs (8-40 x long: at most one kappa change over any
+    8 consecutive x) up to x = noisy_from, then a random value per x."""
+    slow = np.empty((nextrap, nx), np.float32)
+    for l in range(nextrap):
+        x = 0
+        while x < noisy_from:
+            n = int(rng.integers(8, 41))
+            slow[l, x:x + n] = 1.0 / rng.uniform(1500, 4000)
+            x += n
+        slow[l, noisy_from:] = 1.0 / rng.uniform(1500, 4000, nx - noisy_from)
+    return slow
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(), dict(x_per_thread=2, shots_per_group=4), dict(x_per_thread=2, shots_per_group=2, tap_loads=2),
+    dict(x_per_thread=4, shots_per_group=1, tap_loads=2), dict(x_per_thread=4, shots_per_group=1, tap_loads=1),
+    dict(x_per_thread=4, shots_per_group=2), dict(x_per_thread=8, tap_loads=2),
+    dict(x_per_thread=8, tap_loads=1), dict(x_per_thread=8, tables_symmetric=0),
+], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()) or "auto")
+@pytest.mark.parametrize("pattern", ["constant", "blocky", "blocky_then_noise"])
+@pytest.mark.parametrize("conj", [True, False])
+def test_strip_gathers_are_bitwise_the_per_x_gathers(tuning, pattern, conj):
+    """One or two coefficient gathers per strip (wavefronts whose kappa rows
+    change at most once per strip) use the same coefficients as one gather
+    per x: the fields are bitwise equal, and both match the oracle."""
+    import torch
+    from imaging2d_wlsq_amd import device
+    rng = np.random.default_rng(900 + len(pattern) + 7 * len(tuning) + conj)
+    ns, nextrap, nf, nx, M = 4, 5, 6, 1000, 15
+    if tuning.get("x_per_thread") == 8 and pattern == "blocky":
+        nx = 5000  # two strips per thread
+    _, omega, k0, dk, tf, tb = _ktable_problem(rng, nx, M, nf, nextrap, conj=conj)
+    tf = np.ascontiguousarray((tf + tf[:, ::-1]) * 0.5).astype(np.complex64)
+    tb = np.conj(tf).astype(np.complex64) if conj else np.ascontiguousarray(
+        (tb + tb[:, ::-1]) * 0.5).astype(np.complex64)
+    if pattern == "constant":
+        slow = np.full((nextrap, nx), 1.0 / 2500.0, np.float32)
+    else:
+        slow = _blocky_slowness(rng, nextrap, nx, nx if pattern == "blocky" else nx // 2)
+    F = _rand(rng, (ns, nf, nx))
+    B = _rand(rng, (ns, nf, nx))
+    Fo, Bo = F.copy(), B.copy()
+    io = np.zeros((ns, nextrap, nx), np.float32)
+    oracle.extrapolate_ktable(ns, nextrap, nextrap, nx, nf, nf, M, slow, omega, k0, dk, 300, tf, tb, Fo, Bo, io)
+    outs = []
+    for sg in (2, 0):
+        t = dict(tuning, strip_gathers=sg, tables_conjugate=int(conj))
+        print("tuning", t, pattern, "nx", nx, flush=True)
+        dF, dB = torch.from_numpy(F).cuda(), torch.from_numpy(B).cuda()
+        img = torch.zeros((ns, nextrap, nx), device="cuda")
+        device.ktable_device(ns, nextrap, nextrap, nf, nf, nx, M, torch.from_numpy(slow).cuda(),
+                             torch.from_numpy(omega).cuda(), k0, dk, 300, torch.from_numpy(tf).cuda(),
+                             dF, torch.from_numpy(tb).cuda(), dB, img, tuning=t)
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), dF.cpu().numpy(), dB.cpu().numpy()))
+    np.testing.assert_array_equal(outs[1][1], outs[0][1])
+    np.testing.assert_array_equal(outs[1][2], outs[0][2])
+    assert rel(outs[1][0], outs[0][0]) <= 1e-6
+    assert rel(outs[1][0], io) <= RTOL
+    assert crel(outs[1][1], Fo) <= RTOL and crel(outs[1][2], Bo) <= RTOL
+
+
 @pytest.mark.parametrize("layout", [0, 1])
 def test_device_expansion_is_bitwise_the_oracle_expansion(layout):
     import torch
