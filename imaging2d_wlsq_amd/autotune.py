@@ -42,13 +42,15 @@ def candidates(conj: bool, sym: bool, max_nx: int) -> List[dict]:
             for R in (2, 4, 8):
                 if R == 8 and S != 1:
                     continue
-                if (max_nx + R - 1) // R > (1024 if R == 8 else 512):
+                strips = (max_nx + R - 1) // R
+                wide = R == 4 and S == 1  # 1024-thread workgroups, up to 2 strips per thread
+                if strips > (1024 if R == 8 else 2048 if wide else 512):
                     continue
                 if R == 2 and S == 1:
                     continue
-                blocks = [256, 512] if R == 4 else [0]
+                blocks = ([256, 512, 1024] if wide else [256, 512]) if R == 4 else [0]
                 for block in blocks:
-                    if block and block < (max_nx + R - 1) // R:
+                    if block and block * (2 if wide and block > 512 else 1) < strips:
                         continue
                     for W in (0, 1):
                         for tl in ((1, 2) if S <= 2 and R <= 4 else (1,)):

@@ -22,18 +22,13 @@ from . import autotune, device, synth
 
 
 def default_tuning(max_nx: int, shots_share: bool, conj: bool, sym: bool, nshots: int = 1) -> dict:
-    """Launch shape without timing, from an autotuner sweep on MI355X over
-    apertures 512-8192 and 1-16 shots per group (tap pairing wins whenever
-    the tables allow it; two shots per group at 2 x per thread for apertures
-    up to 1024, 4 shots for large groups)."""
-    base = dict(tables_conjugate=int(conj), tables_symmetric=int(sym))
-    if max_nx > 2048:
-        return dict(base, x_per_thread=8, shots_per_group=1)
-    if not shots_share:
-        return dict(base, x_per_thread=4, shots_per_group=1, block=512, omegas_per_item=1)
-    if max_nx <= 1024:
-        return dict(base, x_per_thread=2, shots_per_group=4 if nshots >= 8 else 2)
-    return dict(base, x_per_thread=4, shots_per_group=2, block=512)
+    """Launch shape without timing: the table properties, with the shape left
+    to the library's planner, whose defaults come from an autotuner sweep on
+    MI355X over apertures 256-8192 and 1-16 shots per group (2 x per thread
+    and 2-4 shots for apertures up to 1024 with shared groups, 4 x per thread
+    otherwise, one shot per group and 1024-thread workgroups above 2048)."""
+    del max_nx, shots_share, nshots
+    return dict(tables_conjugate=int(conj), tables_symmetric=int(sym))
 
 
 class Survey:

@@ -306,25 +306,32 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // sweep on MI355X over apertures 512-8192 and 1-16 shots per group
     int R = t.x_per_thread;
     if (R == 0)
-        R = max_nx > 2048 ? 8 : (best_run >= 2 && max_nx <= 1024 && t.shots_per_group != 1) ? 2 : 4;
+        R = (best_run >= 2 && max_nx <= 1024 && t.shots_per_group != 1) ? 2 : 4;
     if (R != 2 && R != 4 && R != 8)
         return fail(WLSQ_EINVAL, "x_per_thread must be 2, 4 or 8 (got %d)", R);
-    const int max_block = 512;
-    const int max_strips = (max_nx + R - 1) / R;
-    // strips per thread: 2 only with 8 x per thread (apertures up to 8192)
-    const int C = max_strips <= max_block ? 1 : 2;
-    if (max_strips > C * max_block || (C == 2 && R != 8))
-        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
-                    max_strips, R, (R == 8 ? 2 : 1) * max_block);
     int S = t.shots_per_group;
     if (S == 0)
-        S = R == 8 ? 1 : R == 2 ? (best_run >= 8 ? 4 : 2) : (best_run >= 2 ? 2 : 1);
+        S = R == 8 ? 1 : R == 2 ? (best_run >= 8 ? 4 : 2) : (best_run >= 2 && max_nx <= 2048 ? 2 : 1);
     if (S != 1 && S != 2 && S != 4)
         return fail(WLSQ_EINVAL, "shots_per_group must be 1, 2 or 4 (got %d)", S);
     if (R == 8 && S != 1)
         return fail(WLSQ_EUNSUPPORTED, "8 x per thread supports 1 shot per group");
     if (R == 2 && S == 1)
         return fail(WLSQ_EUNSUPPORTED, "2 x per thread supports 2 or 4 shots per group");
+    // Single-shot groups of 4 x per thread also run 1024-thread workgroups
+    // (their registers fit four waves per SIMD); the other shapes run up to 512.
+    const int max_block = (R == 4 && S == 1) ? 1024 : 512;
+    const int max_strips = (max_nx + R - 1) / R;
+    // strips per thread: 2 when the aperture has more strips than the
+    // workgroup has threads (8 x per thread, or 4 with 1024 threads)
+    const int threads = t.block > 0 ? t.block : max_block;
+    const int C = max_strips <= threads ? 1 : 2;
+    if (max_strips > C * max_block || (C == 2 && R == 2) || (C == 2 && R == 4 && S != 1))
+        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
+                    max_strips, R, (R == 2 || S != 1 ? 1 : 2) * max_block);
+    if (C == 2 && R == 4 && t.block > 0 && t.block <= 512)
+        return fail(WLSQ_EINVAL, "aperture nx=%d with 4 x per thread needs block > 512 (got %d)",
+                    max_nx, t.block);
 
     int block = t.block;
     if (block == 0) {
@@ -332,7 +339,8 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         if (block < 256 || R == 4)
             block = std::max(block, R == 4 ? 512 : 256);
     }
-    if (block < 64 || block > max_block || block % 64 || block * C < max_strips)
+    if (block < 64 || block > max_block || block % 64 || block * C < max_strips ||
+        (C == 2 && R == 4 && block <= 512))
         return fail(WLSQ_EINVAL, "block=%d invalid for %d strips (multiple of 64, <= %d)", block,
                     max_strips, max_block);
 
@@ -408,8 +416,6 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         int W = block * C / strips;
         if (t.omegas_per_item > 0)
             W = std::min(W, t.omegas_per_item);
-        else if (best_run < 2 && R == 4)
-            W = 1;  // unshared shots: one frequency per task spreads the work widest
         W = std::min(W, s.nf);
         while (W > 1 && fused_lds_bytes(s.nx, M, R, S, W, img_rows) > budget)
             --W;
@@ -442,8 +448,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     }
     if (t.tap_loads < 0 || t.tap_loads > 2)
         return fail(WLSQ_EINVAL, "tap_loads must be 0, 1 or 2 (got %d)", t.tap_loads);
-    // paired tap loads relieve the address unit when no shots share the gathers
-    const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4);
+    // paired tap loads relieve the address unit when no shots share the gathers;
+    // 1024-thread workgroups (128 registers) keep single loads
+    const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4 && block <= 512);
     P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
     P->lds = lds_need;
     P->img_lds = img_lds;

@@ -546,6 +546,15 @@ template <int L_, bool P>
 hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
                           const FusedArgs &a)
 {
+    if (sh.R == 4 && sh.S == 1 && sh.block > 512) {  // 16 waves: registers for 4 per SIMD
+        if (sh.C == 1)
+            return launch_fused_RS<L_, P, 4, 1, 1, 1024>(sh, n_items, lds, st, a);
+        if (sh.C == 2)
+            return launch_fused_RS<L_, P, 4, 1, 2, 1024>(sh, n_items, lds, st, a);
+        return hipErrorInvalidValue;
+    }
+    if (sh.block > 512)
+        return hipErrorInvalidValue;
     if (sh.C == 1) {
         if (sh.R == 4 && sh.S == 1)
             return launch_fused_RS<L_, P, 4, 1, 1, 512>(sh, n_items, lds, st, a);

@@ -217,7 +217,7 @@ hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hip
 {
     if (n_items <= 0)
         return hipSuccess;
-    if (sh.block <= 0 || sh.block > 512 || sh.block % 64 != 0)
+    if (sh.block <= 0 || sh.block > 1024 || sh.block % 64 != 0)
         return hipErrorInvalidValue;
 #define WLSQ_FUSED_BY_P(l)                                                                  \
     return sh.paired ? launch_fused_L##l##_P1(sh, n_items, lds_bytes, st, a)                \

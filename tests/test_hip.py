@@ -190,6 +190,9 @@ def test_ktable_fused_matches_oracle(shape):
     dict(image_mode=2, shots_per_group=4), dict(x_per_thread=2, shots_per_group=4),
     dict(tap_loads=2), dict(tap_loads=2, shots_per_group=1), dict(tap_loads=1, shots_per_group=1),
     dict(tap_loads=2, x_per_thread=8),
+    dict(x_per_thread=4, shots_per_group=1, block=1024),
+    dict(x_per_thread=4, shots_per_group=1, block=1024, tap_loads=2, omegas_per_item=2),
+    dict(x_per_thread=4, shots_per_group=1, block=640),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
@@ -281,6 +284,8 @@ def _blocky_slowness(rng, nextrap, nx, noisy_from):
     dict(x_per_thread=4, shots_per_group=1, tap_loads=2), dict(x_per_thread=4, shots_per_group=1, tap_loads=1),
     dict(x_per_thread=4, shots_per_group=2), dict(x_per_thread=8, tap_loads=2),
     dict(x_per_thread=8, tap_loads=1), dict(x_per_thread=8, tables_symmetric=0),
+    dict(x_per_thread=4, shots_per_group=1, block=1024),
+    dict(x_per_thread=4, shots_per_group=1, block=1024, tables_symmetric=0, tap_loads=2),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()) or "auto")
 @pytest.mark.parametrize("pattern", ["constant", "blocky", "blocky_then_noise"])
 @pytest.mark.parametrize("conj", [True, False])
@@ -292,7 +297,7 @@ def test_strip_gathers_are_bitwise_the_per_x_gathers(tuning, pattern, conj):
     from imaging2d_wlsq_amd import device
     rng = np.random.default_rng(900 + len(pattern) + 7 * len(tuning) + conj)
     ns, nextrap, nf, nx, M = 4, 5, 6, 1000, 15
-    if tuning.get("x_per_thread") == 8 and pattern == "blocky":
+    if (tuning.get("x_per_thread") == 8 or tuning.get("block") == 1024) and pattern == "blocky":
         nx = 5000  # two strips per thread
     _, omega, k0, dk, tf, tb = _ktable_problem(rng, nx, M, nf, nextrap, conj=conj)
     tf = np.ascontiguousarray((tf + tf[:, ::-1]) * 0.5).astype(np.complex64)
@@ -519,7 +524,14 @@ def test_errors_are_reported_not_silent():
     specs[0].ld = 5000
     specs[0].img_ld = 5000
     specs[0].slow_ld = 5000
-    with pytest.raises(_lib.WlsqError):
-        device.Plan(specs, b["M"], dict(x_per_thread=4))  # 1250 strips > 512 threads
+    with pytest.raises(_lib.WlsqError):  # 1250 strips of 2 shots > 512 threads
+        device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=2))
+    with pytest.raises(_lib.WlsqError):  # two strips per thread of 4 x need > 512 threads
+        device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1, block=512))
+    device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1)).close()  # 1024 threads
     with pytest.raises(_lib.WlsqError):
         device.Plan(specs, b["M"], dict(x_per_thread=8, shots_per_group=2))
+    for k in ("nx", "ld", "img_ld", "slow_ld"):
+        setattr(specs[0], k, 9000)
+    with pytest.raises(_lib.WlsqError):  # 2250 strips of 4 x > 2 x 1024 threads
+        device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1))
