@@ -26,6 +26,7 @@ EXPORTS = [
     "extrapolate_ktable", "wlsq_ktable_device", "wlsq_expand_table_device", "wlsq_batch_device",
     "wlsq_plan_create", "wlsq_plan_info", "wlsq_plan_launch", "wlsq_plan_destroy",
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
+    "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device",
 ]
 
 
@@ -103,6 +104,13 @@ def lib():
     L.wlsq_graph_destroy.argtypes = [vp]
     L.wlsq_batch_points.restype = c_longlong
     L.wlsq_batch_points.argtypes = [POINTER(Shot), ci]
+    # transforms: cpp_src/interface.py:36-45 binds the MKL pair the same way
+    L.c64fft1dforw.restype = ci
+    L.c64fft1dforw.argtypes = [_c64p, ci]
+    L.c64fft1dback.restype = ci
+    L.c64fft1dback.argtypes = [_c64p, ci]
+    L.wlsq_trace_spectra_device.restype = ci
+    L.wlsq_trace_spectra_device.argtypes = [ci, ci, vp, ci, vp, ci, vp]
     _lib = L
     return L
 

@@ -29,9 +29,13 @@ LIB = os.path.join(PKG, "libwlsq_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # (source, object, extra flags): the fused kernel is built once per taps
 # (25, 31, 51; 0 = any, runtime loop bounds) and gather width, in parallel
-SOURCES = [("wlsq_kernels.hip", "wlsq_kernels.o", []), ("wlsq_capi.hip", "wlsq_capi.o", [])] + [
+SOURCES = [("wlsq_kernels.hip", "wlsq_kernels.o", []), ("wlsq_capi.hip", "wlsq_capi.o", []),
+           ("wlsq_fft.hip", "wlsq_fft.o", [])] + [
     ("wlsq_fused.hip", f"wlsq_fused_L{l}_P{p}.o", [f"-DWLSQ_L={l}", f"-DWLSQ_PAIRED={p}"])
     for l in (25, 31, 51, 0) for p in (0, 1)]
+
+# hipFFT (rocFFT) for the trace spectra and the FFT drop-ins
+LINK = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", "--offload-arch=gfx950",
@@ -63,7 +67,7 @@ def build(verbose: bool = False) -> str:
             if verbose:
                 print(" ".join(c))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs, *LINK])
     os.replace(tmp, LIB)
     return LIB
 

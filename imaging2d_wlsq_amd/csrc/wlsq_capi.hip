@@ -608,6 +608,17 @@ struct wlsq_graph_s {
     std::shared_ptr<PlanImpl> plan;
 };
 
+int wlsq::set_error(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 extern "C" {
 
 const char *wlsq_last_error(void) { return g_err.c_str(); }

@@ -113,6 +113,9 @@ hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st);
 // compact (nk, L) table -> rows of L + 1 entries (last 0), read by the fused kernel
 hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStream_t st);
 
+// Records the message returned by wlsq_last_error() and returns `code`.
+int set_error(int code, const char *fmt, ...);
+
 // (float)(1.0 / (double)dkappa): the reciprocal every implementation of the
 // kappa lookup uses.
 inline float inv_dkappa(float dkappa) { return (float)(1.0 / (double)dkappa); }

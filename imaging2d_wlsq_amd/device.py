@@ -82,6 +82,26 @@ def expand_table_device(nextrap, nf, nx, M, slowness, omega, kappa0, dkappa, nk,
                                          _ptr(out, c, "out"), int(layout), _stream(stream)))
 
 
+def trace_spectra_device(traces: torch.Tensor, out: Optional[torch.Tensor] = None,
+                         stream=None) -> torch.Tensor:
+    """(nt, nx) complex64 spectra along time of (nt, nx) float32 traces on the
+    GPU: ``scipy.fftpack.fft(traces, axis=0)`` of ``main.py:119-123``. ``out``
+    may be a row-strided view (ld >= nx) into a larger buffer."""
+    if traces.dim() != 2:
+        raise ValueError("traces must be (nt, nx)")
+    nt, nx = traces.shape
+    tp = _ptr(traces, torch.float32, "traces")
+    if out is None:
+        out = torch.empty((nt, nx), dtype=torch.complex64, device=traces.device)
+    if out.dim() != 2 or out.shape[0] < nt or out.shape[1] < nx or out.stride(1) != 1:
+        raise ValueError("out must be (>= nt, >= nx) with unit column stride")
+    if out.dtype != torch.complex64 or out.device.type != "cuda":
+        raise TypeError("out must be complex64 on the GPU")
+    check(lib().wlsq_trace_spectra_device(nt, nx, tp, nx, out.data_ptr(), int(out.stride(0)),
+                                          _stream(stream)))
+    return out
+
+
 def tables_conjugate(table_forw: torch.Tensor, table_back: torch.Tensor) -> bool:
     """True when table_back == conj(table_forw) bit for bit (then the fused
     kernel reads one table)."""

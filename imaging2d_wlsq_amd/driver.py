@@ -15,11 +15,11 @@ Same arguments as ``main.py:30-34``. The steps restate ``main.py``:
 5. Ricker source per shot (``util.py:47-58`` with the call of ``main.py:120``)
    and FFT along time of source and receiver (``main.py:121-123``);
 6. extrapolation and imaging on the GPU:
-   * ``device_hip``: compact tables, operators gathered on the device
-     (``extrapolate_ktable``);
-   * ``device_hip_expanded``: the reference-shaped call - tables expanded to
-     (nextrap, nw, nx, L) on the host as ``makeTableOfOperators`` does
-     (``preparation.py:101-115``) and passed to the drop-in ``extrapolate``;
+   * ``device_hip``: device-resident - trace spectra by hipFFT on the GPU,
+     compact tables, operators gathered on the device (``wlsq_ktable_device``);
+   * ``device_hip_expanded``: the reference-shaped call - host FFTs, tables
+     expanded to (nextrap, nw, nx, L) on the host as ``makeTableOfOperators``
+     does (``preparation.py:101-115``) and passed to the drop-in ``extrapolate``;
 7. sum the per-shot images and save ``final_image.csv`` (``main.py:184-190``).
 
 The ``host`` options of the reference are its CPU library, which this
@@ -152,18 +152,20 @@ def migrate(velocity: np.ndarray, setup_text: str, shot_ix: List[int], shots: Li
     v0 = float(vel[0, 0])  # main.py:111
     tj = [j * cfg.dt for j in range(cfg.nt)]
     xi = [cfg.xmin + (i + 1) * cfg.dx for i in range(nx)]
-    F = np.zeros((ns, cfg.nt, nx), np.complex64)
-    B = np.zeros((ns, cfg.nt, nx), np.complex64)
-    for s in range(ns):
-        src = ricker_shot(cfg.xmin + shot_ix[s] * cfg.dx, cfg.zinit, cfg.nt, nx, tj, xi, cfg.dz, v0)
-        F[s] = spfft.fft(src, axis=0)
-        B[s] = spfft.fft(np.asarray(shots[s], np.float32), axis=0)
-    image = np.zeros((ns, nz, nx), np.float32)
+    sources = [ricker_shot(cfg.xmin + shot_ix[s] * cfg.dx, cfg.zinit, cfg.nt, nx, tj, xi, cfg.dz, v0)
+               for s in range(ns)]
     t1 = time.time()
     if option == "device_hip":
-        interface_hip.extrapolate_ktable(ns, cfg.nextrap, nz, cfg.nt, cfg.nw, nx, cfg.M, slowness,
-                                         omega, kap0, dk, nk, tf, F, tb, B, image)
+        image = _migrate_on_device(cfg, nz, nx, slowness, omega, kap0, dk, nk, tf, tb, sources,
+                                   [np.asarray(r, np.float32) for r in shots])
     elif option == "device_hip_expanded":
+        # the reference-shaped call: host FFTs (main.py:119-123), expanded tables
+        F = np.zeros((ns, cfg.nt, nx), np.complex64)
+        B = np.zeros((ns, cfg.nt, nx), np.complex64)
+        for s in range(ns):
+            F[s] = spfft.fft(sources[s], axis=0)
+            B[s] = spfft.fft(np.asarray(shots[s], np.float32), axis=0)
+        image = np.zeros((ns, nz, nx), np.float32)
         opF = prep.expand_table(tf, slowness, omega, kap0, dk, nk, cfg.nextrap)
         opB = prep.expand_table(tb, slowness, omega, kap0, dk, nk, cfg.nextrap)
         interface_hip.extrapolate(ns, cfg.nextrap, nz, cfg.nt, cfg.nw, nx, cfg.M, opF, F, opB, B, image)
@@ -174,6 +176,32 @@ def migrate(velocity: np.ndarray, setup_text: str, shot_ix: List[int], shots: Li
     for s in range(ns):
         final += image[s]
     return vel, final
+
+
+def _migrate_on_device(cfg: ProblemConfig, nz, nx, slowness, omega, kap0, dk, nk, tf, tb,
+                       sources, records) -> np.ndarray:
+    """Device-resident ``device_hip`` run: real traces are uploaded, their
+    spectra computed on the GPU (``main.py:119-123``), the compact tables
+    gathered on the device; returns the per-shot images (ns, nz, nx)."""
+    import torch
+    from . import autotune, device
+    dev = torch.device("cuda")
+    ns = len(sources)
+    F = torch.empty((ns, cfg.nt, nx), dtype=torch.complex64, device=dev)
+    B = torch.empty_like(F)
+    for s in range(ns):
+        device.trace_spectra_device(torch.from_numpy(np.ascontiguousarray(sources[s])).to(dev), out=F[s])
+        device.trace_spectra_device(torch.from_numpy(np.ascontiguousarray(records[s])).to(dev), out=B[s])
+    tf_d = torch.from_numpy(np.ascontiguousarray(tf)).to(dev)
+    tb_d = torch.from_numpy(np.ascontiguousarray(tb)).to(dev)
+    tuning = dict(tables_conjugate=int(device.tables_conjugate(tf_d, tb_d)),
+                  tables_symmetric=int(autotune.tables_symmetric(tf_d) and autotune.tables_symmetric(tb_d)))
+    image = torch.zeros((ns, nz, nx), dtype=torch.float32, device=dev)
+    device.ktable_device(ns, cfg.nextrap, nz, cfg.nt, cfg.nw, nx, cfg.M,
+                         torch.from_numpy(np.ascontiguousarray(slowness)).to(dev),
+                         torch.from_numpy(np.ascontiguousarray(omega)).to(dev), kap0, dk, nk,
+                         tf_d, F, tb_d, B, image, tuning=tuning)
+    return image.cpu().numpy()
 
 
 def main(argv: Optional[List[str]] = None) -> int:

@@ -84,3 +84,20 @@ def extrapolate_ktable(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, 
     check(lib().extrapolate_ktable(int(ns), int(nextrap), int(nz), int(nt), int(nf), int(nx), int(M),
                                    slowness, omega, float(kappa0), float(dkappa), int(nk),
                                    table_forw, forw_pulse, table_back, back_pulse, image))
+
+
+def c64fft1dforw(data, N):
+    """In-place forward FFT of a complex64 array of length N on the GPU
+    (``cpp_src/interface.py:36-40``, MKL in the reference)."""
+    _c64(data, "data")
+    if data.size < N:
+        raise ValueError(f"data has {data.size} elements, needs {N}")
+    check(lib().c64fft1dforw(data, int(N)))
+
+
+def c64fft1dback(data, N):
+    """In-place backward FFT scaled by 1/N (``cpp_src/interface.py:41-45``)."""
+    _c64(data, "data")
+    if data.size < N:
+        raise ValueError(f"data has {data.size} elements, needs {N}")
+    check(lib().c64fft1dback(data, int(N)))

@@ -214,6 +214,24 @@ int wlsq_graph_destroy(wlsq_graph graph);
 /* Points (x, z, omega, shot) the batched form computes for these shots. */
 long long wlsq_batch_points(const wlsq_shot *shots, int nshots);
 
+/* ---------------------------------------------------------------- transforms
+ * In-place complex FFT of length N of a host array, computed on the device
+ * (hipFFT). Drop-ins for the reference CPU library's MKL wrappers
+ * c64fft1dforw / c64fft1dback (cpp_src/mkl_fft.cpp:9-39, bound in
+ * cpp_src/interface.py:36-45): forward X[k] = sum_n x[n] exp(-2 pi i k n / N);
+ * backward scaled by 1/N. Returns a status instead of void. */
+int c64fft1dforw(wlsq_c32 *data, int N);
+int c64fft1dback(wlsq_c32 *data, int N);
+
+/* Spectra of real traces on the device: out[j * ld_out + x] =
+ * sum_t traces[t * ld_in + x] exp(-2 pi i j t / nt) for all j < nt, x < nx,
+ * i.e. scipy.fftpack.fft(traces, axis=0) of the shot and source records in
+ * main.py:119-123, without the host FFT and the upload of complex spectra.
+ * Rows above nt/2 come from conjugate symmetry. `out` must not overlap
+ * `traces`. Enqueued on `stream` (0 = null stream); plans are cached. */
+int wlsq_trace_spectra_device(int nt, int nx, const float *traces, int ld_in, wlsq_c32 *out,
+                              int ld_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

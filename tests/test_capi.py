@@ -79,3 +79,16 @@ def test_plan_rejects_bad_shots_without_device():
     rc = _lib.lib().wlsq_plan_create(ctypes.byref(h), arr, 1, 2, ctypes.byref(tun))
     # either the shot check (EINVAL) or, with no GPU here, the device query fails first
     assert rc in (_lib.WLSQ_EINVAL, _lib.WLSQ_EHIP)
+
+
+def test_transform_arguments_are_checked_before_the_device():
+    """c64fft1dforw/back (cpp_src/mkl_fft.cpp:9-39 drop-ins) and the trace
+    spectra entry reject bad sizes and strides without a HIP call."""
+    L = _lib.lib()
+    with pytest.raises(_lib.WlsqError) as e:
+        _lib.check(L.c64fft1dforw(np.zeros(4, np.complex64), -1))
+    assert e.value.status == _lib.WLSQ_EINVAL
+    assert L.c64fft1dback(np.zeros(1, np.complex64), 0) == _lib.WLSQ_OK  # empty: no-op
+    assert L.wlsq_trace_spectra_device(4, 8, None, 8, None, 8, None) == _lib.WLSQ_EINVAL
+    assert L.wlsq_trace_spectra_device(4, 8, 1024, 7, 2048, 8, None) == _lib.WLSQ_EINVAL
+    assert L.wlsq_trace_spectra_device(0, 8, None, 8, None, 8, None) == _lib.WLSQ_OK
