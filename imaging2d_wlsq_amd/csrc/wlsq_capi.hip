@@ -318,15 +318,16 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         return fail(WLSQ_EUNSUPPORTED, "8 x per thread supports 1 shot per group");
     if (R == 2 && S == 1)
         return fail(WLSQ_EUNSUPPORTED, "2 x per thread supports 2 or 4 shots per group");
-    // Single-shot groups of 4 x per thread also run 1024-thread workgroups
-    // (their registers fit four waves per SIMD); the other shapes run up to 512.
-    const int max_block = (R == 4 && S == 1) ? 1024 : 512;
+    // Single-shot groups of 4 x per thread and two-shot groups of 2 x also run
+    // 1024-thread workgroups (their registers fit four waves per SIMD); the
+    // other shapes run up to 512.
+    const int max_block = ((R == 4 && S == 1) || (R == 2 && S == 2)) ? 1024 : 512;
     const int max_strips = (max_nx + R - 1) / R;
     // strips per thread: 2 when the aperture has more strips than the
     // workgroup has threads (8 x per thread, or 4 with 1024 threads)
     const int threads = t.block > 0 ? t.block : max_block;
     const int C = max_strips <= threads ? 1 : 2;
-    if (max_strips > C * max_block || (C == 2 && R == 2) || (C == 2 && R == 4 && S != 1))
+    if (max_strips > (R == 2 ? 1 : C) * max_block || (C == 2 && R == 2) || (C == 2 && R == 4 && S != 1))
         return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
                     max_strips, R, (R == 2 || S != 1 ? 1 : 2) * max_block);
     if (C == 2 && R == 4 && t.block > 0 && t.block <= 512)
@@ -336,8 +337,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     int block = t.block;
     if (block == 0) {
         block = (((max_strips + C - 1) / C + 63) / 64) * 64;
-        if (block < 256 || R == 4)
-            block = std::max(block, R == 4 ? 512 : 256);
+        // single-shot groups: 1024 threads (several frequencies per task on
+        // narrow apertures); 4 x per thread with shots: 512; else >= 256
+        block = std::max(block, (R == 4 && S == 1) ? 1024 : R == 4 ? 512 : 256);
     }
     if (block < 64 || block > max_block || block % 64 || block * C < max_strips ||
         (C == 2 && R == 4 && block <= 512))

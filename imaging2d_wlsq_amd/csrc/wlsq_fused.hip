@@ -18,6 +18,19 @@
 namespace wlsq {
 namespace {
 
+// R consecutive floats at p (p is 4R-byte aligned: x0 is a multiple of R)
+template <int R>
+__device__ __forceinline__ void store_row(float *p, const float (&v)[R])
+{
+    if constexpr (R == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r += 4)
+            *reinterpret_cast<float4 *>(p + r) = make_float4(v[r], v[r + 1], v[r + 2], v[r + 3]);
+    }
+}
+
 // ------------------------------------------------------------------ fused kernel
 // LDS holds one 16-byte element (F, B) per position of each padded row, so a
 // window load is one ds_read_b128 for both fields. Position p of a row lives
@@ -68,10 +81,9 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         rows[row * rowlen + swz<R>(p, Q)] = make_float4(vf.x, vf.y, vb.x, vb.y);
     }
     const bool shared_img = gp->img_shared != 0;
-    const int img_rows = shared_img ? 1 : nshots;  // image rows reduced in LDS
-    if (a.img_lds)
-        for (int e = tid; e < img_rows * nxr; e += NT)
-            imgrow[e] = 0.f;
+    // image partial rows in LDS: one per (image, frequency row); every entry
+    // x < nxr is written by its owner thread each step before it is read
+    const int img_rows = shared_img ? 1 : nshots;
 
     long long ioff[S];
 #pragma unroll
@@ -103,6 +115,13 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     __syncthreads();
 
     float2 accF_[C][S][R], accB_[C][S][R];
+    float sv_[C][R];  // one strip per thread: slowness of the coming step at the strip's x
+#pragma unroll
+    for (int cs = 0; cs < C; ++cs) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            sv_[cs][r] = (C == 1 && actv[cs] && nz > 0) ? slow0[min(x0v[cs] + r, nx - 1)] : 0.f;
+    }
     for (int l = 0; l < nz; ++l) {
         // one depth step of strip cs (a compile-time index: the accumulators
         // stay in registers)
@@ -113,12 +132,26 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             float2 (&accB)[S][R] = accB_[cs];
             const int x0 = x0v[cs];
             const float om = omv[cs];
-            const float *srow = slow0 + (long long)l * slow_ld;
+            // One strip per thread: the slowness of this step was loaded during
+            // the previous one, and the next step's load is issued now so its
+            // latency hides behind the taps. Two strips per thread have no
+            // registers to spare and load at the start of the step.
             int qo[R];
+            if constexpr (C == 1) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int xr = min(x0 + r, nx - 1);
-                qo[r] = kappa_index(om, srow[xr], a.kappa0, a.inv_dk, a.nk) * Lp;
+                for (int r = 0; r < R; ++r)
+                    qo[r] = kappa_index(om, sv_[cs][r], a.kappa0, a.inv_dk, a.nk) * Lp;
+                if (l + 1 < nz) {
+                    const float *srow = slow0 + (long long)(l + 1) * slow_ld;
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        sv_[cs][r] = srow[min(x0 + r, nx - 1)];
+                }
+            } else {
+                const float *srow = slow0 + (long long)l * slow_ld;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    qo[r] = kappa_index(om, srow[min(x0 + r, nx - 1)], a.kappa0, a.inv_dk, a.nk) * Lp;
             }
             // Kappa rows are piecewise constant along x. When every active lane
             // of the wavefront has one row over its strip (mode 1), or one
@@ -424,30 +457,32 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 (void)changes;
                 step(std::integral_constant<int, 0>{});
             }
-            // imaging condition of this step. With img_lds the task's rows are
-            // summed in LDS (over its frequencies, and over its shots when they
-            // share an image) and flushed once per (depth, x) below.
+            // imaging condition of this step. With img_lds the thread stores
+            // its R values (summed over the shots when they share an image)
+            // in the partial row of its frequency; the flush below sums the
+            // task's frequency rows in fixed order, one atomic per (depth, x).
             if (a.img_lds) {
+                const int w = wv[cs];
                 if (shared_img) {
+                    float v[R];
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        float v = 0.f;
+                        v[r] = 0.f;
 #pragma unroll
                         for (int s = 0; s < S; ++s)
                             if (s < nshots)
-                                v += re_mul_conj(accF[s][r], accB[s][r]);
-                        if (x0 + r < nx)
-                            atomicAdd(&imgrow[x0 + r], v);
+                                v[r] += re_mul_conj(accF[s][r], accB[s][r]);
                     }
+                    store_row<R>(imgrow + (size_t)w * nxr + x0, v);
                 } else {
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         if (s < nshots) {
+                            float v[R];
 #pragma unroll
                             for (int r = 0; r < R; ++r)
-                                if (x0 + r < nx)
-                                    atomicAdd(&imgrow[s * nxr + x0 + r],
-                                              re_mul_conj(accF[s][r], accB[s][r]));
+                                v[r] = re_mul_conj(accF[s][r], accB[s][r]);
+                            store_row<R>(imgrow + ((size_t)s * W + w) * nxr + x0, v);
                         }
                     }
                 }
@@ -495,8 +530,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             for (int e = tid; e < img_rows * nx; e += NT) {
                 const int s = e / nx;
                 const int x = e - s * nx;
-                const float v = imgrow[s * nxr + x];
-                imgrow[s * nxr + x] = 0.f;
+                const float *pr = imgrow + (size_t)s * W * nxr + x;
+                float v = pr[0];
+                for (int w = 1; w < W; ++w)
+                    v += pr[(size_t)w * nxr];
                 unsafeAtomicAdd(a.img + gp->img_off[s] + (long long)l * img_ld + x, v);
             }
         }
@@ -553,6 +590,8 @@ hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStre
             return launch_fused_RS<L_, P, 4, 1, 2, 1024>(sh, n_items, lds, st, a);
         return hipErrorInvalidValue;
     }
+    if (sh.R == 2 && sh.S == 2 && sh.C == 1 && sh.block > 512)
+        return launch_fused_RS<L_, P, 2, 2, 1, 1024>(sh, n_items, lds, st, a);
     if (sh.block > 512)
         return hipErrorInvalidValue;
     if (sh.C == 1) {

@@ -71,7 +71,7 @@ struct FusedShape {
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of
-// R sub-rows, plus `img_rows` image rows of the padded aperture.
+// R sub-rows, plus `img_rows` x W image partial rows of the padded aperture.
 size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows);
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,
