@@ -225,6 +225,7 @@ struct PlanImpl {
     size_t lds = 0;
     int img_lds = 1;
     int strip_gathers = 1;
+    int double_rows = 0;
     long long points = 0;
     std::vector<Group> groups;
     std::vector<Item> items;
@@ -454,11 +455,22 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // 1024-thread workgroups (128 registers) keep single loads
     const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4 && block <= 512);
     P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
-    P->lds = lds_need;
     P->img_lds = img_lds;
     if (t.strip_gathers < 0 || t.strip_gathers > 2)
         return fail(WLSQ_EINVAL, "strip_gathers must be 0, 1 or 2 (got %d)", t.strip_gathers);
     P->strip_gathers = t.strip_gathers == 2 ? 0 : 1;
+    // Two copies of the rows (one barrier per step) when every task's doubled
+    // footprint fits the device's LDS.
+    if (t.double_rows < 0 || t.double_rows > 2)
+        return fail(WLSQ_EINVAL, "double_rows must be 0, 1 or 2 (got %d)", t.double_rows);
+    if (t.double_rows != 2 && 2 * lds_need <= lds_max) {
+        P->double_rows = 1;
+        lds_need *= 2;
+    } else if (t.double_rows == 1) {
+        return fail(WLSQ_EUNSUPPORTED, "two copies of the rows need %zu bytes of LDS (device max %zu)",
+                    2 * lds_need, lds_max);
+    }
+    P->lds = lds_need;
     if (!P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
@@ -506,6 +518,7 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     a.write_back = write_back;
     a.img_lds = P.img_lds;
     a.strip_gathers = P.strip_gathers;
+    a.double_rows = P.double_rows;
     HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
     return WLSQ_OK;
 }
@@ -554,7 +567,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
                                t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
-                               t.strip_gathers, (long long)(intptr_t)st};  // a plan's table copies are per stream
+                               t.strip_gathers, t.double_rows, (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);

@@ -62,10 +62,15 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const int nrows = S * W;
     if (W * nstrips > C * NT)
         return;  // the planner never builds such a task
+    // With double_rows the task has two copies of its rows and of its image
+    // partial rows: step l reads copy l % 2 and writes copy (l + 1) % 2, so
+    // one barrier per step separates the reads from the next writes.
+    const int nbuf = a.double_rows ? 2 : 1;
+    const size_t rows_size = (size_t)nrows * rowlen;  // 16-byte elements per copy
     float4 *rows = lds4;  // row (s * W + w): shot s, frequency j0 + w
-    float *imgrow = reinterpret_cast<float *>(lds4 + (size_t)nrows * rowlen);
+    float *imgrow0 = reinterpret_cast<float *>(lds4 + nbuf * rows_size);
 
-    // Initial rows: interior from the fields, zero halos and padding.
+    // Initial rows (every copy): interior from the fields, zero halos and padding.
     for (int e = tid; e < nrows * rowlen; e += NT) {
         const int row = e / rowlen;
         const int p = e - row * rowlen;
@@ -78,12 +83,16 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             vf = a.F[gp->f_off[s] + r];
             vb = a.B[gp->b_off[s] + r];
         }
-        rows[row * rowlen + swz<R>(p, Q)] = make_float4(vf.x, vf.y, vb.x, vb.y);
+        const float4 v4 = make_float4(vf.x, vf.y, vb.x, vb.y);
+        rows[row * rowlen + swz<R>(p, Q)] = v4;
+        if (nbuf == 2)
+            rows[rows_size + row * rowlen + swz<R>(p, Q)] = v4;
     }
     const bool shared_img = gp->img_shared != 0;
     // image partial rows in LDS: one per (image, frequency row); every entry
     // x < nxr is written by its owner thread each step before it is read
     const int img_rows = shared_img ? 1 : nshots;
+    const size_t img_size = (size_t)img_rows * W * nxr;  // floats per copy
 
     long long ioff[S];
 #pragma unroll
@@ -123,6 +132,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             sv_[cs][r] = (C == 1 && actv[cs] && nz > 0) ? slow0[min(x0v[cs] + r, nx - 1)] : 0.f;
     }
     for (int l = 0; l < nz; ++l) {
+        const int cur = nbuf == 2 ? (l & 1) : 0;
+        const float4 *rows_r = rows + cur * rows_size;               // read this step
+        float4 *rows_w = rows + (nbuf == 2 ? cur ^ 1 : 0) * rows_size; // written this step
+        float *imgrow = imgrow0 + cur * img_size;
         // one depth step of strip cs (a compile-time index: the accumulators
         // stay in registers)
         auto strip_step = [&](auto cs_c) {
@@ -183,7 +196,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             for (int s = 0; s < S; ++s) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    bs[s][r] = rows + (size_t)s * shot_rows + (size_t)wv[cs] * rowlen + r * Q + xs;
+                    bs[s][r] = rows_r + (size_t)s * shot_rows + (size_t)wv[cs] * rowlen + r * Q + xs;
             }
             auto ldw = [&](int s, int j) { return bs[s][j & (R - 1)][j >> LOG2R]; };
             auto step = [&](auto mode_c) {
@@ -502,15 +515,8 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             }
         }
         };
-        strip_step(std::integral_constant<int, 0>{});
-        if constexpr (C == 2)
-            strip_step(std::integral_constant<int, 1>{});
-        static_assert(C == 1 || C == 2, "one or two strips per thread");
-        __syncthreads();
-#pragma unroll
-        for (int cs = 0; cs < C; ++cs) {
-            if (!actv[cs])
-                continue;
+        auto store_strip = [&](auto cs_c) {
+            constexpr int cs = decltype(cs_c)::value;
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -518,11 +524,30 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                     const int x = x0v[cs] + r;
                     if (x < nx) {
                         const float2 f = accF_[cs][s][r], b = accB_[cs][s][r];
-                        rows[s * shot_rows + (size_t)wv[cs] * rowlen + swz<R>(x + M, Q)] =
+                        rows_w[s * shot_rows + (size_t)wv[cs] * rowlen + swz<R>(x + M, Q)] =
                             make_float4(f.x, f.y, b.x, b.y);
                     }
                 }
             }
+        };
+        using c0 = std::integral_constant<int, 0>;
+        using c1 = std::integral_constant<int, 1>;
+        strip_step(c0{});
+        if (nbuf == 2 && actv[0])
+            store_strip(c0{});
+        if constexpr (C == 2) {
+            strip_step(c1{});
+            if (nbuf == 2 && actv[1])
+                store_strip(c1{});
+        }
+        static_assert(C == 1 || C == 2, "one or two strips per thread");
+        __syncthreads();
+        if (nbuf == 1) {
+            if (actv[0])
+                store_strip(c0{});
+            if constexpr (C == 2)
+                if (actv[1])
+                    store_strip(c1{});
         }
         if (a.img_lds) {
             // consecutive lanes flush consecutive x: one 32-byte sector of the
@@ -537,17 +562,20 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 unsafeAtomicAdd(a.img + gp->img_off[s] + (long long)l * img_ld + x, v);
             }
         }
-        __syncthreads();
+        if (nbuf == 1)
+            __syncthreads();
     }
 
     if (a.write_back) {
+        // after the last step's barrier; with two copies the last step wrote copy nz % 2
+        const float4 *fin = rows + (nbuf == 2 ? (nz & 1) : 0) * rows_size;
         for (int e = tid; e < nrows * nx; e += NT) {
             const int row = e / nx;
             const int x = e - row * nx;
             const int s = row / W;
             const int ww = row - s * W;
             if (s < nshots) {
-                const float4 v = rows[row * rowlen + swz<R>(x + M, Q)];
+                const float4 v = fin[row * rowlen + swz<R>(x + M, Q)];
                 const long long r = (long long)(it.j0 + ww) * ld + x;
                 a.F[gp->f_off[s] + r] = make_float2(v.x, v.y);
                 a.B[gp->b_off[s] + r] = make_float2(v.z, v.w);

@@ -57,6 +57,7 @@ struct FusedArgs {
     int write_back;
     int img_lds;  // 1: reduce image rows in LDS, flush with lane-contiguous atomics
     int strip_gathers;  // 1: one or two gathers per strip where the kappa rows allow it
+    int double_rows;    // 1: two copies of the rows in LDS, one barrier per step
 };
 
 // Launch shape of the fused kernel; chosen by the planner.
@@ -71,8 +72,9 @@ struct FusedShape {
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of
-// R sub-rows, plus `img_rows` x W image partial rows of the padded aperture.
-size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows);
+// R sub-rows, plus `img_rows` x W image partial rows of the padded aperture;
+// `copies` = 2 doubles both (one barrier per depth step).
+size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows, int copies = 1);
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,
                         const FusedArgs &a);

@@ -203,13 +203,14 @@ __global__ void fill_f32_kernel(float *p, long long n, float v)
 
 }  // namespace
 
-size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows)
+size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows, int copies)
 {
     const size_t nstrips = (size_t)(nx + R - 1) / R;
     const size_t nxr = nstrips * R;
     const size_t Q = (nxr + 2 * (size_t)M + R) / R;
     const size_t rowlen = Q * R;
-    return 2 * (size_t)S * W * rowlen * sizeof(float2) + (size_t)img_rows * W * nxr * sizeof(float);
+    return (size_t)copies * (2 * (size_t)S * W * rowlen * sizeof(float2) +
+                             (size_t)img_rows * W * nxr * sizeof(float));
 }
 
 hipError_t launch_fused(const FusedShape &sh, int n_items, size_t lds_bytes, hipStream_t st,

@@ -131,6 +131,10 @@ typedef struct {
                              over its x_per_thread positions, gather each tap
                              once or twice per lane instead of once per x;
                              2 = off. Results are bitwise the same. */
+    int double_rows;      /* 0 = auto (on when the tasks' doubled LDS fits the
+                             device), 1 = on (error if it does not fit), 2 =
+                             off: two copies of the rows in LDS, one barrier
+                             per depth step instead of two */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
