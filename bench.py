@@ -224,7 +224,11 @@ def main():
     ap.add_argument("--omegas-per-item", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--x-per-thread", type=int, default=0)
-    ap.add_argument("--no-tune", action="store_true", help="use the planner's default shape")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="uneven-batch leg: use the planner's default shape instead of the autotuner")
+    ap.add_argument("--tune-c2", action="store_true",
+                    help="main leg: autotune the launch shape (default: the planner's shape, which "
+                         "the committed profile covers)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=200,
                     help="jobs of the serving run (0: skip; the full configuration is 2000)")
@@ -286,7 +290,7 @@ def main():
     B = B0.clone()
     image = torch.zeros((nz, nx), dtype=torch.float32, device=dev)
     forced = any([args.shots_per_group, args.omegas_per_item, args.block, args.x_per_thread])
-    if args.no_tune or forced:
+    if not args.tune_c2 or forced:
         tuning = dict(tables_conjugate=int(conj), tables_symmetric=int(sym),
                       shots_per_group=args.shots_per_group, omegas_per_item=args.omegas_per_item,
                       block=args.block, x_per_thread=args.x_per_thread)

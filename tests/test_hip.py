@@ -193,6 +193,9 @@ def test_ktable_fused_matches_oracle(shape):
     dict(x_per_thread=4, shots_per_group=1, block=1024),
     dict(x_per_thread=4, shots_per_group=1, block=1024, tap_loads=2, omegas_per_item=2),
     dict(x_per_thread=4, shots_per_group=1, block=640),
+    dict(double_rows=2), dict(double_rows=1, shots_per_group=1),
+    dict(double_rows=2, x_per_thread=4, shots_per_group=1, block=1024),
+    dict(x_per_thread=2, shots_per_group=2, block=1024),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
@@ -535,3 +538,5 @@ def test_errors_are_reported_not_silent():
         setattr(specs[0], k, 9000)
     with pytest.raises(_lib.WlsqError):  # 2250 strips of 4 x > 2 x 1024 threads
         device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1))
+    with pytest.raises(_lib.WlsqError):  # 9000-wide rows twice do not fit 160 KB of LDS
+        device.Plan(specs, b["M"], dict(x_per_thread=8, shots_per_group=1, double_rows=1))
