@@ -463,7 +463,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // footprint fits the device's LDS.
     if (t.double_rows < 0 || t.double_rows > 2)
         return fail(WLSQ_EINVAL, "double_rows must be 0, 1 or 2 (got %d)", t.double_rows);
-    if (t.double_rows != 2 && 2 * lds_need <= lds_max) {
+    if (t.double_rows != 2 && C == 1 && 2 * lds_need <= lds_max) {
         P->double_rows = 1;
         lds_need *= 2;
     } else if (t.double_rows == 1) {

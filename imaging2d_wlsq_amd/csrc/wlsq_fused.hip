@@ -65,7 +65,8 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     // With double_rows the task has two copies of its rows and of its image
     // partial rows: step l reads copy l % 2 and writes copy (l + 1) % 2, so
     // one barrier per step separates the reads from the next writes.
-    const int nbuf = a.double_rows ? 2 : 1;
+    // (two strips per thread means rows of more than 4096 positions: one copy)
+    const int nbuf = (C == 1 && a.double_rows) ? 2 : 1;
     const size_t rows_size = (size_t)nrows * rowlen;  // 16-byte elements per copy
     float4 *rows = lds4;  // row (s * W + w): shot s, frequency j0 + w
     float *imgrow0 = reinterpret_cast<float *>(lds4 + nbuf * rows_size);
@@ -552,14 +553,15 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         if (a.img_lds) {
             // consecutive lanes flush consecutive x: one 32-byte sector of the
             // image carries 8 atomic adds
-            for (int e = tid; e < img_rows * nx; e += NT) {
-                const int s = e / nx;
-                const int x = e - s * nx;
-                const float *pr = imgrow + (size_t)s * W * nxr + x;
-                float v = pr[0];
-                for (int w = 1; w < W; ++w)
-                    v += pr[(size_t)w * nxr];
-                unsafeAtomicAdd(a.img + gp->img_off[s] + (long long)l * img_ld + x, v);
+            for (int s = 0; s < img_rows; ++s) {
+                float *dst = a.img + gp->img_off[s] + (long long)l * img_ld;
+                const float *src = imgrow + (size_t)s * W * nxr;
+                for (int x = tid; x < nx; x += NT) {
+                    float v = src[x];
+                    for (int w = 1; w < W; ++w)
+                        v += src[(size_t)w * nxr + x];
+                    unsafeAtomicAdd(dst + x, v);
+                }
             }
         }
         if (nbuf == 1)
