@@ -2,8 +2,9 @@
 
 For a batch of shots it times candidate tunings - shots per group (coefficient
 reuse), x per thread (register blocking), threads per workgroup, frequencies
-per task (LDS rows per workgroup) and the tap-pairing variant when the tables
-allow it - with HIP events on the caller's stream, and returns the fastest.
+per task (LDS rows per workgroup), the tap-pairing variant when the tables
+allow it, then depth steps per launch for the best of them - with HIP events
+on the caller's stream, and returns the fastest.
 Results are cached in memory and, optionally, in a JSON file keyed by the
 workload signature (apertures, depths, frequency counts, taps, device name).
 
@@ -112,6 +113,28 @@ def tune(shots: Sequence[device.ShotSpec], M: int, slowness, omega, kappa0, dkap
         raise RuntimeError("no launch shape fits this batch")
     results.sort(key=lambda r: r[0])
     best = results[0][1]
+    # depth steps per launch: all of them, or the range split in 2 or 4
+    # launches (each restarts from the written-back fields)
+    max_nz = max(s.nz for s in shots)
+    if cands is None and max_nz >= 64:
+        for parts in (2, 4):
+            c = dict(best, depth_steps=(max_nz + parts - 1) // parts)
+            try:
+                plan = device.Plan(shots, M, c)
+            except WlsqError:
+                continue
+            try:
+                ms = _time_plan(plan, args, reps, stream)
+            except WlsqError:
+                plan.close()
+                continue
+            info = plan.info()
+            plan.close()
+            if verbose:
+                print(f"  tune {c} -> {ms:.3f} ms {info}", flush=True)
+            if ms < results[0][0]:
+                results.insert(0, (ms, c, info))
+        best = results[0][1]
     _MEM[key] = best
     if cache_file:
         try:

@@ -226,6 +226,10 @@ struct PlanImpl {
     int img_lds = 1;
     int strip_gathers = 1;
     int double_rows = 0;
+    int depth_steps = 0;         // depth steps per launch (0: all)
+    int max_nz = 0;
+    long long f_extent = 0, b_extent = 0;  // elements of the F and B pools the shots use
+    DevBuf fields;               // copies of the pools for split launches without write-back
     long long points = 0;
     std::vector<Group> groups;
     std::vector<Item> items;
@@ -471,6 +475,17 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
                     2 * lds_need, lds_max);
     }
     P->lds = lds_need;
+    if (t.depth_steps < 0)
+        return fail(WLSQ_EINVAL, "depth_steps must be >= 0 (got %d)", t.depth_steps);
+    P->depth_steps = t.depth_steps;
+    for (int i = 0; i < nshots; ++i) {
+        const wlsq_shot &sh = shots[i];
+        if (sh.nx == 0 || sh.nf == 0 || sh.nz == 0)
+            continue;
+        P->max_nz = std::max(P->max_nz, sh.nz);
+        P->f_extent = std::max(P->f_extent, sh.f_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
+        P->b_extent = std::max(P->b_extent, sh.b_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
+    }
     if (!P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
@@ -519,7 +534,31 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     a.img_lds = P.img_lds;
     a.strip_gathers = P.strip_gathers;
     a.double_rows = P.double_rows;
-    HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+    const int ds = P.depth_steps;
+    if (ds == 0 || ds >= P.max_nz) {
+        a.l0 = 0;
+        a.l1 = P.max_nz;
+        HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+        return WLSQ_OK;
+    }
+    // Depth range split over launches of ds steps: each launch starts from
+    // the fields the previous one wrote back. Without write-back the caller's
+    // fields stay untouched: the launches run on copies.
+    if (!write_back) {
+        WLSQ_TRY(P.fields.reserve(sizeof(float2) * (size_t)(P.f_extent + P.b_extent)));
+        float2 *cf = P.fields.as<float2>();
+        float2 *cb = cf + P.f_extent;
+        HIP_TRY(hipMemcpyAsync(cf, forw, sizeof(float2) * (size_t)P.f_extent, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(cb, back, sizeof(float2) * (size_t)P.b_extent, hipMemcpyDeviceToDevice, st));
+        a.F = cf;
+        a.B = cb;
+    }
+    a.write_back = 1;
+    for (int l0 = 0; l0 < P.max_nz; l0 += ds) {
+        a.l0 = l0;
+        a.l1 = l0 + ds;
+        HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+    }
     return WLSQ_OK;
 }
 
@@ -567,7 +606,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
                                t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
-                               t.strip_gathers, t.double_rows, (long long)(intptr_t)st};  // a plan's table copies are per stream
+                               t.strip_gathers, t.double_rows, t.depth_steps, (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -946,8 +985,15 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
     *out = nullptr;
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
+    // no allocation inside the capture: table copies, and the field copies of
+    // split launches without write-back
     if (nk > 0)
-        WLSQ_TRY(plan->impl->reserve_tables(nk));  // no allocation inside the capture
+        WLSQ_TRY(plan->impl->reserve_tables(nk));
+    {
+        PlanImpl &P = *plan->impl;
+        if (!write_back && P.depth_steps > 0 && P.depth_steps < P.max_nz)
+            WLSQ_TRY(P.fields.reserve(sizeof(float2) * (size_t)(P.f_extent + P.b_extent)));
+    }
     hipStream_t cap = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     auto g = std::make_unique<wlsq_graph_s>();

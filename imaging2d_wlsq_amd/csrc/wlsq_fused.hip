@@ -62,6 +62,12 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const int nrows = S * W;
     if (W * nstrips > C * NT)
         return;  // the planner never builds such a task
+    // depth steps of this launch: [lb, le) of the task's nz (a plan may split
+    // the depth range over several launches; the fields carry the state)
+    const int lb = a.l0;
+    const int le = min(nz, a.l1);
+    if (lb >= le)
+        return;  // this task's depth range ended in an earlier launch
     // With double_rows the task has two copies of its rows and of its image
     // partial rows: step l reads copy l % 2 and writes copy (l + 1) % 2, so
     // one barrier per step separates the reads from the next writes.
@@ -130,10 +136,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     for (int cs = 0; cs < C; ++cs) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            sv_[cs][r] = (C == 1 && actv[cs] && nz > 0) ? slow0[min(x0v[cs] + r, nx - 1)] : 0.f;
+            sv_[cs][r] = (C == 1 && actv[cs]) ? slow0[(long long)lb * slow_ld + min(x0v[cs] + r, nx - 1)] : 0.f;
     }
-    for (int l = 0; l < nz; ++l) {
-        const int cur = nbuf == 2 ? (l & 1) : 0;
+    for (int l = lb; l < le; ++l) {
+        const int cur = nbuf == 2 ? ((l - lb) & 1) : 0;
         const float4 *rows_r = rows + cur * rows_size;               // read this step
         float4 *rows_w = rows + (nbuf == 2 ? cur ^ 1 : 0) * rows_size; // written this step
         float *imgrow = imgrow0 + cur * img_size;
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     qo[r] = kappa_index(om, sv_[cs][r], a.kappa0, a.inv_dk, a.nk) * Lp;
-                if (l + 1 < nz) {
+                if (l + 1 < le) {
                     const float *srow = slow0 + (long long)(l + 1) * slow_ld;
 #pragma unroll
                     for (int r = 0; r < R; ++r)
@@ -569,8 +575,9 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     }
 
     if (a.write_back) {
-        // after the last step's barrier; with two copies the last step wrote copy nz % 2
-        const float4 *fin = rows + (nbuf == 2 ? (nz & 1) : 0) * rows_size;
+        // after the last step's barrier; with two copies the last step wrote
+        // copy (le - lb) % 2
+        const float4 *fin = rows + (nbuf == 2 ? ((le - lb) & 1) : 0) * rows_size;
         for (int e = tid; e < nrows * nx; e += NT) {
             const int row = e / nx;
             const int x = e - row * nx;

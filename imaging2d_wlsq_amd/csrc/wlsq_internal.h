@@ -58,6 +58,7 @@ struct FusedArgs {
     int img_lds;  // 1: reduce image rows in LDS, flush with lane-contiguous atomics
     int strip_gathers;  // 1: one or two gathers per strip where the kappa rows allow it
     int double_rows;    // 1: two copies of the rows in LDS, one barrier per step
+    int l0, l1;         // depth steps [l0, min(nz, l1)) of each task in this launch
 };
 
 // Launch shape of the fused kernel; chosen by the planner.

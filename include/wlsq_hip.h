@@ -135,6 +135,11 @@ typedef struct {
                              device), 1 = on (error if it does not fit), 2 =
                              off: two copies of the rows in LDS, one barrier
                              per depth step instead of two */
+    int depth_steps;      /* 0 = every depth step in one launch; n > 0 = launches
+                             of n steps, each continuing from the fields the
+                             previous one wrote (without write-back they run
+                             on device copies of the fields: the caller's stay
+                             untouched) */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

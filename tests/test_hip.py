@@ -196,6 +196,7 @@ def test_ktable_fused_matches_oracle(shape):
     dict(double_rows=2), dict(double_rows=1, shots_per_group=1),
     dict(double_rows=2, x_per_thread=4, shots_per_group=1, block=1024),
     dict(x_per_thread=2, shots_per_group=2, block=1024),
+    dict(depth_steps=2), dict(depth_steps=4, double_rows=2), dict(depth_steps=1, shots_per_group=1),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed

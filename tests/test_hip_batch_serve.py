@@ -110,6 +110,30 @@ def test_survey_batch_matches_oracle(graph):
     b.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_depth_split_launches_match_one_launch(graph):
+    """The depth range split over launches of 7 steps (fields carried through
+    device copies, the caller's fields untouched) gives the one-launch image,
+    also when captured in a graph and replayed."""
+    import torch
+    from imaging2d_wlsq_amd import batch
+    p = _small_problem()
+    shots = _uneven_shots(p, 5, 33)
+    sv = batch.Survey(p)
+    b1 = sv.batch(shots)
+    ref = b1.run().clone()
+    torch.cuda.synchronize()
+    b2 = sv.batch(shots, tuning=dict(b1.tuning, depth_steps=7), graph=graph)
+    F0, B0 = b2.F.clone(), b2.B.clone()
+    for rep in range(3):
+        img = b2.run().clone()
+        torch.cuda.synchronize()
+        assert rel(img.cpu().numpy(), ref.cpu().numpy()) <= 1e-6, rep
+    assert torch.equal(b2.F, F0) and torch.equal(b2.B, B0)
+    b1.close()
+    b2.close()
+
+
 def test_each_job_same_alone_and_in_the_mix():
     import torch
     from imaging2d_wlsq_amd import serve
