@@ -201,8 +201,10 @@ int wlsq_plan_create(wlsq_plan *out, const wlsq_shot *shots, int nshots, int M,
 int wlsq_plan_info(wlsq_plan plan, long long *points, int *n_items, int *block, int *lds_bytes,
                    int *x_per_thread, int *shots_per_group);
 /* Queue the plan's launch on `stream` (a row-padded copy of the tables into
- * the plan's storage, then the fused kernel); adds into `image`. Launches of
- * one plan on different streams must not overlap. */
+ * the plan's storage, then the fused kernel, or one kernel per depth range
+ * with tuning.depth_steps); adds into `image`. Launches of one plan on
+ * different streams must not overlap: the table copies, and the field copies
+ * of split launches without write-back, belong to the plan. */
 int wlsq_plan_launch(wlsq_plan plan, const float *slowness, const float *omega, float kappa0,
                      float dkappa, int nk, const wlsq_c32 *table_forw,
                      const wlsq_c32 *table_back, wlsq_c32 *forw, wlsq_c32 *back, float *image,
