@@ -194,7 +194,8 @@ def bench_serving(args, log):
     mean_pts = float(np.mean([j.points for j in probe]))
     rate = args.serve_load * cap_pts / mean_pts
     jobs = serve.make_jobs(args.serve_jobs, rate, len(models), nzm, nxm, seed=4)
-    st = serve.Server(launcher, hbm_budget=budget).run(jobs)
+    st = serve.Server(launcher, hbm_budget=budget).run(
+        jobs, log=log if args.serve_jobs >= 500 else None, log_every=100)
     launcher.close()
     s = st.summary()
     s.update({"arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,

@@ -186,8 +186,10 @@ class Server:
         self.hbm_budget = int(hbm_budget)
 
     def run(self, jobs: Sequence[Job], keep_images: bool = False, time_scale: float = 1.0,
-            log=None, clock=time.perf_counter, idle=lambda: time.sleep(2e-4)) -> ServeStats:
-        """Serve `jobs` (arrival times in seconds, divided by `time_scale`)."""
+            log=None, log_every: int = 1, clock=time.perf_counter,
+            idle=lambda: time.sleep(2e-4)) -> ServeStats:
+        """Serve `jobs` (arrival times in seconds, divided by `time_scale`);
+        `log` gets a line every `log_every` completed jobs."""
         st = ServeStats()
         for j in jobs:
             if j.footprint() > self.hbm_budget:
@@ -217,9 +219,9 @@ class Server:
                     free_slots.append(a.slot)
                     active.remove(a)
                     progressed = True
-                    if log:
-                        log(f"job {a.job.job_id} done: {a.job.points:.3g} pts, "
-                            f"latency {st.latencies[-1]:.3f} s")
+                    if log and len(st.latencies) % max(1, log_every) == 0:
+                        log(f"job {a.job.job_id} done ({len(st.latencies)} of {len(jobs)}): "
+                            f"{a.job.points:.3g} pts, latency {st.latencies[-1]:.3f} s")
             while waiting and free_slots:
                 job = waiting[0]
                 need = job.footprint()
