@@ -92,3 +92,24 @@ def test_transform_arguments_are_checked_before_the_device():
     assert L.wlsq_trace_spectra_device(4, 8, None, 8, None, 8, None) == _lib.WLSQ_EINVAL
     assert L.wlsq_trace_spectra_device(4, 8, 1024, 7, 2048, 8, None) == _lib.WLSQ_EINVAL
     assert L.wlsq_trace_spectra_device(0, 8, None, 8, None, 8, None) == _lib.WLSQ_OK
+
+
+@pytest.mark.parametrize("empty", [dict(ns=0), dict(nextrap=0), dict(nx=0)])
+def test_empty_inputs_are_a_no_op_before_the_device(empty):
+    """No shots, no depth steps or no aperture: nothing to compute, status OK,
+    outputs untouched (the reference loops simply do not run)."""
+    a = dict(ns=1, nextrap=1, nz=1, nt=2, nf=2, nx=4, M=1)
+    a.update(empty)
+    op = np.zeros((1, 2, 4, 3), np.complex64)
+    F = np.full((1, 2, 4), 1 + 2j, np.complex64)
+    B = F.copy()
+    img = np.full((1, 1, 4), 5.0, np.float32)
+    L = _lib.lib()
+    assert L.extrapolate(a["ns"], a["nextrap"], a["nz"], a["nt"], a["nf"], a["nx"], a["M"],
+                         op, F, op, B, img) == _lib.WLSQ_OK
+    sl = np.ones((1, 4), np.float32)
+    om = np.ones(2, np.float32)
+    tab = np.zeros((3, 3), np.complex64)
+    assert L.extrapolate_ktable(a["ns"], a["nextrap"], a["nz"], a["nt"], a["nf"], a["nx"], a["M"],
+                                sl, om, 0.0, 1.0, 3, tab, F, tab, B, img) == _lib.WLSQ_OK
+    assert (F == 1 + 2j).all() and (img == 5.0).all()
