@@ -381,7 +381,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
                 return false;
         return true;
     };
-    // image rows reduced in LDS unless some group cannot fit them at W = 1
+    // image partial rows in LDS unless some group cannot fit them at W = 1
     if (t.image_mode < 0 || t.image_mode > 2)
         return fail(WLSQ_EINVAL, "image_mode must be 0, 1 or 2 (got %d)", t.image_mode);
     int img_lds = t.image_mode == 2 ? 0 : 1;

@@ -108,17 +108,20 @@ int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
 typedef struct {
     int shots_per_group; /* 0 = auto, else 1, 2 or 4 shots sharing coefficient loads */
     int omegas_per_item; /* 0 = auto: frequencies per workgroup task */
-    int block;           /* 0 = auto: threads per workgroup (64..512) */
+    int block;           /* 0 = auto: threads per workgroup, a multiple of 64:
+                            up to 1024 for one shot per group at 4 x per
+                            thread and for two shots at 2 x, else up to 512 */
     int x_per_thread;    /* 0 = auto: consecutive x per thread, 2 (with 2 or 4
-                            shots per group), 4 or 8 (1 shot per group);
-                            apertures up to 512 * x_per_thread, 8192 with 8
-                            (two strips per thread) */
+                            shots per group), 4 or 8 (8 with 1 shot per
+                            group); a task's aperture is covered by one or two
+                            strips of x_per_thread per thread (two: 4 x at
+                            more than 512 threads, or 8 x), up to 8192 */
     int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
     int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
                              bit for bit, so only table_forw is read */
-    int image_mode;       /* 0 = auto, 1 = image rows reduced in LDS then one
-                             atomic add per (depth, x) per task, 2 = an atomic
-                             add per point */
+    int image_mode;       /* 0 = auto, 1 = per-frequency image partial rows in
+                             LDS, summed in fixed order, then one atomic add per
+                             (depth, x) per task, 2 = an atomic add per point */
     int tables_symmetric; /* 1: caller guarantees both tables are tap-symmetric,
                              T[q][k] == T[q][2M-k] bit for bit (the WLSQ design
                              with symmetric taps gives this); taps are then
