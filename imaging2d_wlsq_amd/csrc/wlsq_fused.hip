@@ -1,10 +1,12 @@
 // gfx950 fused extrapolate-and-image kernel (compact operator tables).
 //
 // A workgroup owns complete rows (S shots x W frequencies) of one shot group
-// in LDS for ALL depth steps, gathers each operator from a compact (nk x L)
-// table by the kappa lookup and adds the image with atomics. Semantics as in
-// py_src/wavefieldPropagation/extrapolation.py:57-64 and
-// cpp_src/extrapolation_cpu.cpp:39-89 (unconjugated dot, taps in order).
+// in LDS for all its depth steps, gathers each operator from a compact
+// (nk x L) table by the kappa lookup and adds the image with atomics.
+// Semantics as in py_src/wavefieldPropagation/extrapolation.py:57-64 and
+// cpp_src/extrapolation_cpu.cpp:39-89: an unconjugated dot over the taps,
+// summed in tap order, or, for tap-symmetric tables, as M tap pairs plus the
+// centre tap (the same sum in a different rounding order).
 //
 // This file is compiled once per (WLSQ_L, WLSQ_PAIRED): taps 25, 31, 51 or 0
 // (any L, runtime loop bounds) and 8- or 16-byte coefficient gathers, so the
