@@ -40,7 +40,8 @@ class Tuning(Structure):
     _fields_ = [("shots_per_group", c_int), ("omegas_per_item", c_int), ("block", c_int),
                 ("x_per_thread", c_int), ("lds_budget", c_int), ("tables_conjugate", c_int),
                 ("image_mode", c_int), ("tables_symmetric", c_int), ("tap_loads", c_int),
-                ("strip_gathers", c_int), ("double_rows", c_int), ("depth_steps", c_int)]
+                ("strip_gathers", c_int), ("double_rows", c_int), ("depth_steps", c_int),
+                ("image_planes", c_int)]
 
 
 class Shot(Structure):

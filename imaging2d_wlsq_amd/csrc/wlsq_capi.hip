@@ -224,6 +224,10 @@ struct PlanImpl {
     FusedShape shape{4, 1, 1, 256, false, false, false};
     size_t lds = 0;
     int img_lds = 1;
+    int img_xpose = 0;
+    int nplanes = 1;             // image planes of the direct atomics
+    long long img_extent = 0;    // floats of the image the shots touch
+    DevBuf planes;
     int strip_gathers = 1;
     int double_rows = 0;
     int depth_steps = 0;         // depth steps per launch (0: all)
@@ -381,10 +385,13 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
                 return false;
         return true;
     };
-    // image partial rows in LDS unless some group cannot fit them at W = 1
-    if (t.image_mode < 0 || t.image_mode > 2)
-        return fail(WLSQ_EINVAL, "image_mode must be 0, 1 or 2 (got %d)", t.image_mode);
-    int img_lds = t.image_mode == 2 ? 0 : 1;
+    // image partial rows in LDS unless some group cannot fit them at W = 1;
+    // then direct atomics, made lane-contiguous through a wave-private LDS
+    // segment in single-frequency tasks (mode 3), or per point as computed
+    // (mode 2)
+    if (t.image_mode < 0 || t.image_mode > 3)
+        return fail(WLSQ_EINVAL, "image_mode must be 0, 1, 2 or 3 (got %d)", t.image_mode);
+    int img_lds = t.image_mode >= 2 ? 0 : 1;
     for (auto &rn : runs) {
         if (!img_lds)
             break;
@@ -460,6 +467,13 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4 && block <= 512);
     P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
     P->img_lds = img_lds;
+    // the transpose and the image planes are compiled for the one-shot
+    // 1024-thread shapes (the default for apertures whose partial rows do not
+    // fit); other shapes add each point into the image
+    const bool xp_shape = R == 4 && S == 1 && block > 512;
+    P->img_xpose = !img_lds && t.image_mode != 2 && xp_shape;
+    // the transpose segments: 64 x R floats per wavefront, one copy
+    const size_t seg_bytes = P->img_xpose ? sizeof(float) * (size_t)block * R : 0;
     if (t.strip_gathers < 0 || t.strip_gathers > 2)
         return fail(WLSQ_EINVAL, "strip_gathers must be 0, 1 or 2 (got %d)", t.strip_gathers);
     P->strip_gathers = t.strip_gathers == 2 ? 0 : 1;
@@ -467,13 +481,17 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // footprint fits the device's LDS.
     if (t.double_rows < 0 || t.double_rows > 2)
         return fail(WLSQ_EINVAL, "double_rows must be 0, 1 or 2 (got %d)", t.double_rows);
-    if (t.double_rows != 2 && C == 1 && 2 * lds_need <= lds_max) {
+    if (t.double_rows != 2 && C == 1 && 2 * lds_need + seg_bytes <= lds_max) {
         P->double_rows = 1;
         lds_need *= 2;
     } else if (t.double_rows == 1) {
         return fail(WLSQ_EUNSUPPORTED, "two copies of the rows need %zu bytes of LDS (device max %zu)",
-                    2 * lds_need, lds_max);
+                    2 * lds_need + seg_bytes, lds_max);
     }
+    lds_need += seg_bytes;
+    if (lds_need > lds_max)
+        return fail(WLSQ_EUNSUPPORTED, "rows and transpose segments need %zu bytes of LDS (device max %zu)",
+                    lds_need, lds_max);
     P->lds = lds_need;
     if (t.depth_steps < 0)
         return fail(WLSQ_EINVAL, "depth_steps must be >= 0 (got %d)", t.depth_steps);
@@ -485,6 +503,21 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         P->max_nz = std::max(P->max_nz, sh.nz);
         P->f_extent = std::max(P->f_extent, sh.f_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
         P->b_extent = std::max(P->b_extent, sh.b_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
+        P->img_extent = std::max(P->img_extent, sh.img_off + (long long)(sh.nz - 1) * sh.img_ld + sh.nx);
+    }
+    // Image planes for the direct atomics: auto = 8 while they take at most
+    // 512 MiB, halved until they fit (none below 2).
+    if (t.image_planes < 0 || t.image_planes > kMaxImagePlanes)
+        return fail(WLSQ_EINVAL, "image_planes must be 0..%d (got %d)", kMaxImagePlanes,
+                    t.image_planes);
+    if (P->img_xpose) {
+        int K = t.image_planes;
+        if (K == 0) {
+            K = 8;
+            while (K >= 2 && (double)K * (double)P->img_extent * sizeof(float) > (double)(1ll << 29))
+                K /= 2;
+        }
+        P->nplanes = K >= 2 ? K : 1;
     }
     if (!P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
@@ -532,6 +565,16 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     a.L = P.L;
     a.write_back = write_back;
     a.img_lds = P.img_lds;
+    a.img_xpose = P.img_xpose;
+    if (P.nplanes > 1) {
+        WLSQ_TRY(P.planes.reserve(sizeof(float) * (size_t)P.nplanes * (size_t)P.img_extent));
+        a.planes = P.planes.as<float>();
+        a.plane_stride = P.img_extent;
+        a.nplanes = P.nplanes;
+        HIP_TRY(launch_fill_f32(a.planes, (long long)P.nplanes * P.img_extent, 0.f, st));
+    } else {
+        a.nplanes = 1;
+    }
     a.strip_gathers = P.strip_gathers;
     a.double_rows = P.double_rows;
     const int ds = P.depth_steps;
@@ -539,6 +582,8 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
         a.l0 = 0;
         a.l1 = P.max_nz;
         HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+        if (P.nplanes > 1)
+            HIP_TRY(launch_planes_reduce(a.planes, a.plane_stride, a.nplanes, image, P.img_extent, st));
         return WLSQ_OK;
     }
     // Depth range split over launches of ds steps: each launch starts from
@@ -559,6 +604,8 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
         a.l1 = l0 + ds;
         HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
     }
+    if (P.nplanes > 1)
+        HIP_TRY(launch_planes_reduce(a.planes, a.plane_stride, a.nplanes, image, P.img_extent, st));
     return WLSQ_OK;
 }
 
@@ -606,7 +653,8 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
     std::vector<long long> key{ns, nextrap, nz, nt, nf, nx, M, slow_ld, t.shots_per_group,
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
                                t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
-                               t.strip_gathers, t.double_rows, t.depth_steps, (long long)(intptr_t)st};  // a plan's table copies are per stream
+                               t.strip_gathers, t.double_rows, t.depth_steps, t.image_planes,
+                               (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -993,6 +1041,8 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
         PlanImpl &P = *plan->impl;
         if (!write_back && P.depth_steps > 0 && P.depth_steps < P.max_nz)
             WLSQ_TRY(P.fields.reserve(sizeof(float2) * (size_t)(P.f_extent + P.b_extent)));
+        if (P.nplanes > 1)
+            WLSQ_TRY(P.planes.reserve(sizeof(float) * (size_t)P.nplanes * (size_t)P.img_extent));
     }
     hipStream_t cap = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));

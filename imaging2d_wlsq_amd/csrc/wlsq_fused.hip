@@ -33,12 +33,22 @@ __device__ __forceinline__ void store_row(float *p, const float (&v)[R])
     }
 }
 
+// Orders LDS accesses of the lanes of one wavefront around it (the hardware
+// executes a wavefront's LDS instructions in order; this keeps the compiler
+// from moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------ fused kernel
 // LDS holds one 16-byte element (F, B) per position of each padded row, so a
 // window load is one ds_read_b128 for both fields. Position p of a row lives
 // at sub-row (p mod R), slot p / R (swz): lanes that read p = R*lane + c for a
 // fixed c touch consecutive 16-byte slots.
-template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP>
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
     constexpr int LOG2R = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
@@ -98,6 +108,15 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             rows[rows_size + row * rowlen + swz<R>(p, Q)] = v4;
     }
     const bool shared_img = gp->img_shared != 0;
+    // XP (direct atomics, compiled for the one-shot 1024-thread shapes): the
+    // values of single-frequency tasks go through the wave-private transpose
+    // (a wavefront's slots are consecutive strips of the one row), and
+    // consecutive tasks add into different image planes, since tasks running
+    // side by side would all hit the same image row at the same depth.
+    const bool xpose = XP && W == 1;
+    float *const imgd = (XP && a.nplanes > 1)
+                            ? a.planes + (long long)(blockIdx.x % a.nplanes) * a.plane_stride
+                            : a.img;
     // image partial rows in LDS: one per (image, frequency row); every entry
     // x < nxr is written by its owner thread each step before it is read
     const int img_rows = shared_img ? 1 : nshots;
@@ -149,6 +168,17 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         // stay in registers)
         auto strip_step = [&](auto cs_c) {
         constexpr int cs = decltype(cs_c)::value;
+        // this step's imaging values for the wave-private transpose (zero on
+        // lanes without a strip, which take part in the exchange)
+        float xv[S][R];
+        if constexpr (XP) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    xv[s][r] = 0.f;
+            }
+        }
         if (actv[cs]) {
             float2 (&accF)[S][R] = accF_[cs];
             float2 (&accB)[S][R] = accB_[cs];
@@ -508,6 +538,14 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         }
                     }
                 }
+            } else if (xpose) {
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (s < nshots)
+                            xv[s][r] = re_mul_conj(accF[s][r], accB[s][r]);
+                }
             } else {
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
@@ -516,12 +554,56 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         for (int r = 0; r < R; ++r) {
                             const int x = x0 + r;
                             if (x < nx)
-                                unsafeAtomicAdd(a.img + ioff[s] + (long long)l * img_ld + x,
+                                unsafeAtomicAdd(imgd + ioff[s] + (long long)l * img_ld + x,
                                                 re_mul_conj(accF[s][r], accB[s][r]));
                         }
                     }
                 }
             }
+        }
+        if constexpr (XP) {
+        if (xpose) {
+            // Direct atomics of a single-frequency task: the wavefront's 64
+            // strips are 64*R consecutive x from x_w. Lane i stores its R values
+            // at seg[i*R ...], then lane j adds seg[j + 64k] to x = x_w + j + 64k
+            // (k < R): each atomic instruction covers 64 consecutive floats.
+            // The bound is re-read each step (the empty asm keeps the compiler
+            // from hoisting the R compares out of the depth loop, where their
+            // masks would occupy registers through the taps).
+            const int lane = tid & 63;
+            float *seg = imgrow0 + (tid >> 6) * (64 * R);
+            const int xl = ((tid & ~63) + cs * NT) * R + lane;  // x of k = 0
+            int nxl = nx;
+            asm volatile("" : "+v"(nxl));
+            const long long roff = (long long)l * img_ld;
+            const int nimg = shared_img ? 1 : nshots;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (s < nimg) {
+                    float v[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if (shared_img) {
+                            v[r] = xv[0][r];
+#pragma unroll
+                            for (int t = 1; t < S; ++t)
+                                v[r] += xv[t][r];
+                        } else {
+                            v[r] = xv[s][r];
+                        }
+                    }
+                    wave_sync();  // the previous reads of seg are done
+                    store_row<R>(seg + lane * R, v);
+                    wave_sync();
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        const float u = seg[lane + 64 * k];
+                        if (xl + 64 * k < nxl)
+                            unsafeAtomicAdd(imgd + ioff[s] + roff + xl + 64 * k, u);
+                    }
+                }
+            }
+        }
         }
         };
         auto store_strip = [&](auto cs_c) {
@@ -595,10 +677,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     }
 }
 
-template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP>
 hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
 {
-    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT>;
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP>;
     hipError_t e = ensure_lds(k, lds);
     if (e != hipSuccess)
         return e;
@@ -606,16 +688,16 @@ hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, co
     return hipGetLastError();
 }
 
-template <int L_, bool P, int R, int S, int C, int MAXT>
+template <int L_, bool P, int R, int S, int C, int MAXT, bool XP = false>
 hipError_t launch_fused_RS(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
                            const FusedArgs &a)
 {
     const int b = sh.block;
     if (sh.sym)
-        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT>(n_items, b, lds, st, a);
-    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT>(n_items, b, lds, st, a)
-                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT>(n_items, b, lds, st, a);
+        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT, XP>(n_items, b, lds, st, a)
+                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT, XP>(n_items, b, lds, st, a);
+    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT, XP>(n_items, b, lds, st, a)
+                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT, XP>(n_items, b, lds, st, a);
 }
 
 template <int L_, bool P>
@@ -623,10 +705,13 @@ hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStre
                           const FusedArgs &a)
 {
     if (sh.R == 4 && sh.S == 1 && sh.block > 512) {  // 16 waves: registers for 4 per SIMD
+        const bool xp = !a.img_lds && a.img_xpose;
         if (sh.C == 1)
-            return launch_fused_RS<L_, P, 4, 1, 1, 1024>(sh, n_items, lds, st, a);
+            return xp ? launch_fused_RS<L_, P, 4, 1, 1, 1024, true>(sh, n_items, lds, st, a)
+                      : launch_fused_RS<L_, P, 4, 1, 1, 1024>(sh, n_items, lds, st, a);
         if (sh.C == 2)
-            return launch_fused_RS<L_, P, 4, 1, 2, 1024>(sh, n_items, lds, st, a);
+            return xp ? launch_fused_RS<L_, P, 4, 1, 2, 1024, true>(sh, n_items, lds, st, a)
+                      : launch_fused_RS<L_, P, 4, 1, 2, 1024>(sh, n_items, lds, st, a);
         return hipErrorInvalidValue;
     }
     if (sh.R == 2 && sh.S == 2 && sh.C == 1 && sh.block > 512)

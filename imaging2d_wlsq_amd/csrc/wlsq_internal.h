@@ -11,6 +11,7 @@
 namespace wlsq {
 
 constexpr int kMaxShotsPerGroup = 4;
+constexpr int kMaxImagePlanes = 16;
 
 // One shot group: up to 4 shots that share aperture, depth count and
 // frequency list, so one coefficient load serves all of them.
@@ -56,6 +57,12 @@ struct FusedArgs {
     int L;
     int write_back;
     int img_lds;  // 1: reduce image rows in LDS, flush with lane-contiguous atomics
+    int img_xpose;  // img_lds 0: single-frequency tasks transpose each wavefront's values
+                    // through a wave-private LDS segment (lane-contiguous atomics)
+    float *planes;  // img_lds 0 and nplanes > 1: task b adds its direct atomics into
+                    // plane b % nplanes (laid out like img), summed into img afterwards
+    long long plane_stride;
+    int nplanes;
     int strip_gathers;  // 1: one or two gathers per strip where the kappa rows allow it
     int double_rows;    // 1: two copies of the rows in LDS, one barrier per step
     int l0, l1;         // depth steps [l0, min(nz, l1)) of each task in this launch
@@ -65,7 +72,7 @@ struct FusedArgs {
 struct FusedShape {
     int R;      // consecutive x per thread (2, 4, 8)
     int S;      // shots per group (1, 2, 4)
-    int C;      // strips per thread (1, or 2 with R = 8)
+    int C;      // strips per thread (1, or 2: R = 8, or R = 4 above 512 threads)
     int block;  // threads per workgroup
     bool conj;  // table_back == conj(table_forw): load one table
     bool sym;   // tables are tap-symmetric (T[k] == T[L-1-k]): pair the taps
@@ -113,6 +120,10 @@ hipError_t launch_expand(const float *slow, int slow_ld, const float *omega, flo
                          float inv_dk, int nk, const float2 *table, float2 *out, int nextrap,
                          int nf, int nx, int L, int layout, hipStream_t st);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+// img[i] += sum over c < nplanes of planes[c * stride + i] (fixed order), for
+// i < n where the sum is nonzero
+hipError_t launch_planes_reduce(const float *planes, long long stride, int nplanes, float *img,
+                                long long n, hipStream_t st);
 // compact (nk, L) table -> rows of L + 1 entries (last 0), read by the fused kernel
 hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStream_t st);
 

@@ -201,6 +201,21 @@ __global__ void fill_f32_kernel(float *p, long long n, float v)
         p[e] = v;
 }
 
+// Image planes of the direct atomics summed into the image, plane 0 first.
+// Positions no task touched sum to zero and keep their bits.
+__global__ void planes_reduce_kernel(const float *__restrict__ planes, long long pstride,
+                                     int nplanes, float *img, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += stride) {
+        float s = 0.f;
+        for (int c = 0; c < nplanes; ++c)
+            s += planes[c * pstride + e];
+        if (s != 0.f)
+            img[e] += s;
+    }
+}
+
 }  // namespace
 
 size_t fused_lds_bytes(int nx, int M, int R, int S, int W, int img_rows, int copies)
@@ -349,6 +364,19 @@ hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStr
         return hipSuccess;
     hipLaunchKernelGGL(table_pad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out,
                        nk, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_planes_reduce(const float *planes, long long stride, int nplanes, float *img,
+                                long long n, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    hipLaunchKernelGGL(planes_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, planes,
+                       stride, nplanes, img, n);
     return hipGetLastError();
 }
 

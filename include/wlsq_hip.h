@@ -119,9 +119,18 @@ typedef struct {
     int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
     int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
                              bit for bit, so only table_forw is read */
-    int image_mode;       /* 0 = auto, 1 = per-frequency image partial rows in
-                             LDS, summed in fixed order, then one atomic add per
-                             (depth, x) per task, 2 = an atomic add per point */
+    int image_mode;       /* 0 = auto (1 where the partial rows fit next to the
+                             field rows, else 3), 1 = per-frequency image
+                             partial rows in LDS, summed in fixed order, then one
+                             atomic add per (depth, x) per task, 2 = an atomic add
+                             per point from the thread that computed it, 3 = as 2,
+                             but with one shot per group, 4 x per thread and
+                             more than 512 threads, single-frequency tasks first
+                             transpose each wavefront's values through a
+                             wave-private LDS segment (64 x 4 floats per
+                             wavefront), so each atomic instruction covers
+                             consecutive x, and the tasks add into image planes
+                             (image_planes); other shapes add as in 2 */
     int tables_symmetric; /* 1: caller guarantees both tables are tap-symmetric,
                              T[q][k] == T[q][2M-k] bit for bit (the WLSQ design
                              with symmetric taps gives this); taps are then
@@ -143,6 +152,15 @@ typedef struct {
                              previous one wrote (without write-back they run
                              on device copies of the fields: the caller's stay
                              untouched) */
+    int image_planes;     /* direct atomics of image_mode 3 (also auto where
+                             the partial rows do not fit): 0 = auto (8 planes
+                             while they take at most 512 MiB, halved until they
+                             fit, none below 2), 1 = none, n <= 16 = n planes.
+                             Consecutive tasks add into different image-shaped
+                             planes of plan-owned device memory, zeroed on the
+                             launch stream and summed into the image after the
+                             launch, so tasks running side by side do not
+                             contend for the same image addresses. */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

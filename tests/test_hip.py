@@ -197,6 +197,13 @@ def test_ktable_fused_matches_oracle(shape):
     dict(double_rows=2, x_per_thread=4, shots_per_group=1, block=1024),
     dict(x_per_thread=2, shots_per_group=2, block=1024),
     dict(depth_steps=2), dict(depth_steps=4, double_rows=2), dict(depth_steps=1, shots_per_group=1),
+    dict(image_mode=3), dict(image_mode=3, omegas_per_item=1),
+    dict(image_mode=3, omegas_per_item=1, shots_per_group=4),
+    dict(image_mode=3, omegas_per_item=1, x_per_thread=4, shots_per_group=1, block=1024),
+    dict(image_mode=3, omegas_per_item=1, x_per_thread=8),
+    dict(image_mode=3, omegas_per_item=1, x_per_thread=2, shots_per_group=2),
+    dict(image_mode=3, omegas_per_item=1, double_rows=2, shots_per_group=1),
+    dict(image_mode=3, omegas_per_item=1, depth_steps=2, shots_per_group=1),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 def test_ktable_device_tunings_agree(tuning):
     """Every launch shape computes the same fields bit for bit (taps are summed
@@ -290,6 +297,7 @@ def _blocky_slowness(rng, nextrap, nx, noisy_from):
     dict(x_per_thread=8, tap_loads=1), dict(x_per_thread=8, tables_symmetric=0),
     dict(x_per_thread=4, shots_per_group=1, block=1024),
     dict(x_per_thread=4, shots_per_group=1, block=1024, tables_symmetric=0, tap_loads=2),
+    dict(x_per_thread=4, shots_per_group=1, block=1024, image_mode=3, omegas_per_item=1),
 ], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()) or "auto")
 @pytest.mark.parametrize("pattern", ["constant", "blocky", "blocky_then_noise"])
 @pytest.mark.parametrize("conj", [True, False])

@@ -110,6 +110,36 @@ def test_survey_batch_matches_oracle(graph):
     b.close()
 
 
+@pytest.mark.parametrize("tuning", [
+    dict(image_mode=3, omegas_per_item=1), dict(image_mode=3, omegas_per_item=1, shots_per_group=1),
+    dict(image_mode=3, omegas_per_item=1, x_per_thread=2, shots_per_group=4),
+    dict(image_mode=2, omegas_per_item=1)], ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
+def test_direct_image_atomics_match_partial_rows(tuning):
+    """Shot groups that share one image (same window and frequencies): the
+    direct atomics, per point or through the wave-private transpose, give the
+    image of the LDS partial rows (to rounding of the atomic order) and the
+    same final fields bit for bit."""
+    import torch
+    from imaging2d_wlsq_amd import batch, synth
+    p = _small_problem(seed=12, nz=30, nx=700)
+    base = _uneven_shots(p, 3, 44)
+    shots = [synth.ShotGeom(s.x0, s.nx, s.nz, s.xs + 0.5 * k, s.freqs, s.seed + k)
+             for s in base for k in range(4)]
+    sv = batch.Survey(p)
+    outs = []
+    for t in (dict(image_mode=1), tuning):
+        t = dict(t, tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym))
+        b = sv.batch(shots, tuning=t, write_back=True)
+        print("tuning", t, b.plan.info(), flush=True)
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), b.F.cpu().numpy(), b.B.cpu().numpy()))
+        b.close()
+    assert rel(outs[1][0], outs[0][0]) <= 1e-6
+    np.testing.assert_array_equal(outs[1][1], outs[0][1])
+    np.testing.assert_array_equal(outs[1][2], outs[0][2])
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_depth_split_launches_match_one_launch(graph):
     """The depth range split over launches of 7 steps (fields carried through
