@@ -541,6 +541,11 @@ def test_errors_are_reported_not_silent():
     with pytest.raises(_lib.WlsqError):  # two strips per thread of 4 x need > 512 threads
         device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1, block=512))
     device.Plan(specs, b["M"], dict(x_per_thread=4, shots_per_group=1)).close()  # 1024 threads
+    for bad in (-1, 17):
+        with pytest.raises(_lib.WlsqError):
+            device.Plan(specs, b["M"], dict(image_mode=3, image_planes=bad))
+    with pytest.raises(_lib.WlsqError):
+        device.Plan(specs, b["M"], dict(image_mode=4))
     with pytest.raises(_lib.WlsqError):
         device.Plan(specs, b["M"], dict(x_per_thread=8, shots_per_group=2))
     for k in ("nx", "ld", "img_ld", "slow_ld"):

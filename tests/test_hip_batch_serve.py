@@ -141,6 +141,37 @@ def test_direct_image_atomics_match_partial_rows(tuning):
 
 
 @pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("planes", [0, 1, 3, 16])
+def test_image_planes_match_one_image(planes, graph):
+    """Direct atomics of single-frequency tasks through the wave-private
+    transpose into 0 (auto: 8), 1 (the image itself), 3 or 16 image planes,
+    over one launch or depth-split launches, eager or replayed as a graph:
+    the image of the LDS partial rows, to rounding of the atomic order."""
+    import torch
+    from imaging2d_wlsq_amd import batch
+    p = _small_problem(seed=13, nz=30, nx=900)
+    shots = _uneven_shots(p, 6, 55)
+    sv = batch.Survey(p)
+    base = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym))
+    b0 = sv.batch(shots, tuning=dict(base, image_mode=1))
+    ref = b0.run().clone()
+    torch.cuda.synchronize()
+    b0.close()
+    t = dict(base, image_mode=3, x_per_thread=4, shots_per_group=1, block=1024, omegas_per_item=1,
+             image_planes=planes, depth_steps=7 if planes == 3 else 0)
+    b = sv.batch(shots, tuning=t, graph=graph)
+    print("tuning", t, b.plan.info(), flush=True)
+    for rep in range(3):
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        assert rel(img.cpu().numpy(), ref.cpu().numpy()) <= 1e-6, rep
+        junk = torch.randn(1000, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        del junk
+    b.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
 def test_depth_split_launches_match_one_launch(graph):
     """The depth range split over launches of 7 steps (fields carried through
     device copies, the caller's fields untouched) gives the one-launch image,
