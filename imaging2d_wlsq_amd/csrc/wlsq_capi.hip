@@ -250,6 +250,10 @@ struct PlanImpl {
             (void)hipFree(d_items);
         if (d_tt)
             (void)hipFree(d_tt);
+        if (fields.p)
+            (void)hipFree(fields.p);
+        if (planes.p)
+            (void)hipFree(planes.p);
     }
     // table copy storage for nk samples (allocates: call outside graph capture)
     int reserve_tables(int nk)

@@ -171,6 +171,35 @@ def test_image_planes_match_one_image(planes, graph):
     b.close()
 
 
+def test_plan_device_buffers_are_freed():
+    """Plans with image planes and depth-split field copies give their device
+    memory back when destroyed."""
+    import torch
+    from imaging2d_wlsq_amd import batch
+    p = _small_problem(seed=14, nz=30, nx=900)
+    shots = _uneven_shots(p, 4, 66)
+    sv = batch.Survey(p)
+    t = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym), image_mode=3, x_per_thread=4,
+             shots_per_group=1, block=1024, omegas_per_item=1, image_planes=16, depth_steps=7)
+
+    def once():
+        b = sv.batch(shots, tuning=t)
+        b.run()
+        torch.cuda.synchronize()
+        b.close()
+        del b
+
+    once()
+    torch.cuda.empty_cache()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(30):
+        once()
+    torch.cuda.empty_cache()
+    free1 = torch.cuda.mem_get_info()[0]
+    print("free before", free0, "after", free1, flush=True)
+    assert free0 - free1 < (16 << 20)
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_depth_split_launches_match_one_launch(graph):
     """The depth range split over launches of 7 steps (fields carried through
