@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 METRIC = "extrapolated points/s (nx·nz·nf·shots) batched vs plain; p50/p99 job latency"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# vector L1 of 256 CUs at 64 B per clock and 2.4 GHz: the ceiling for the
+# compact-table coefficient gathers, which HBM does not see (the tables stay in L2)
+L1_PEAK_TBS = 256 * 64 * 2.4e9 / 1e12
 
 
 def log(*a):
@@ -94,6 +97,17 @@ def cpu_baseline(p, nshots_sample=16):
     return {"value": pts / dt, "unit": "points/s", "cores": cores, "kind": "port",
             "sample": f"{len(shots)} of {len(p.shots)} shots, all {m.nz} depth steps, nf {s.nf}, "
                       f"nx {m.nx} ({pts:.3g} points, {dt:.1f} s, compact-table mode, {cpu})"}
+
+
+def onchip_coefficients(M, L, conj, sym, shots_per_group, points, kernel_ms):
+    """The other ceiling of the compact-table kernel: coefficient bytes gathered
+    from L1/L2 per point (M + 1 taps per x with tap-symmetric tables, else L;
+    one table when they are conjugate; shared by the shots of a group)."""
+    taps = (M + 1) if sym else L
+    per_point = taps * 8 * (1 if conj else 2) / max(1, shots_per_group)
+    tbs = per_point * points / (kernel_ms / 1e3) / 1e12
+    return {"what": "coefficient gathers (vector L1/L2)", "bytes_per_point": per_point,
+            "achieved": tbs, "peak": L1_PEAK_TBS, "unit": "TB/s", "frac": tbs / L1_PEAK_TBS}
 
 
 def bench_c3(args, log):
@@ -358,7 +372,9 @@ def main():
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                      "traffic": read_profile_traffic("fused_kernel_c2_512x1024_nf256_16shots.txt"),
                      "kernel": "fused_kernel (L=31)", "kernel_ms": k_avg,
-                     "flops_per_point": flops_per_point(L)},
+                     "flops_per_point": flops_per_point(L),
+                     "onchip": onchip_coefficients(t.M, L, conj, sym, info["shots_per_group"],
+                                                   points, k_avg)},
     }
 
     # ---------------------------------------------------------------- plain
