@@ -2,7 +2,11 @@
 //
 // A workgroup owns complete rows (S shots x W frequencies) of one shot group
 // in LDS for all its depth steps, gathers each operator from a compact
-// (nk x L) table by the kappa lookup and adds the image with atomics.
+// (nk x L) table by the kappa lookup and adds the image with atomics: from
+// per-frequency partial rows in LDS, or, where those do not fit next to the
+// field rows, straight from the threads (the XP variant of the one-shot
+// 1024-thread shapes: lane-contiguous through a wave-private LDS transpose,
+// into image planes that the host sums afterwards).
 // Semantics as in py_src/wavefieldPropagation/extrapolation.py:57-64 and
 // cpp_src/extrapolation_cpu.cpp:39-89: an unconjugated dot over the taps,
 // summed in tap order, or, for tap-symmetric tables, as M tap pairs plus the
