@@ -6,7 +6,7 @@
 // per-frequency partial rows in LDS, or, where those do not fit next to the
 // field rows, straight from the threads (the XP variant of the one-shot
 // 1024-thread shapes: lane-contiguous through a wave-private LDS transpose,
-// into image planes that the host sums afterwards).
+// into image planes that a reduction kernel adds to the image after the launch).
 // Semantics as in py_src/wavefieldPropagation/extrapolation.py:57-64 and
 // cpp_src/extrapolation_cpu.cpp:39-89: an unconjugated dot over the taps,
 // summed in tap order, or, for tap-symmetric tables, as M tap pairs plus the
