@@ -1037,8 +1037,8 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
     *out = nullptr;
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
-    // no allocation inside the capture: table copies, and the field copies of
-    // split launches without write-back
+    // no allocation inside the capture: table copies, the field copies of
+    // split launches without write-back, and the image planes
     if (nk > 0)
         WLSQ_TRY(plan->impl->reserve_tables(nk));
     {
