@@ -466,9 +466,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     }
     if (t.tap_loads < 0 || t.tap_loads > 2)
         return fail(WLSQ_EINVAL, "tap_loads must be 0, 1 or 2 (got %d)", t.tap_loads);
-    // paired tap loads relieve the address unit when no shots share the gathers;
-    // 1024-thread workgroups (128 registers) keep single loads
-    const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4 && block <= 512);
+    // paired tap loads relieve the address unit when no shots share the gathers
+    // (1024-thread workgroups too, now that their tap loop is fenced: 4-14 %)
+    const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4);
     P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
     P->img_lds = img_lds;
     // the transpose and the image planes are compiled for the one-shot

@@ -136,8 +136,9 @@ typedef struct {
                              with symmetric taps gives this); taps are then
                              paired: (old[x+k-M] + old[x+M-k]) * T[k], which
                              changes the rounding order, not the operator */
-    int tap_loads;        /* 0 = auto, 1 = one tap per coefficient load, 2 = two
-                             taps per 16-byte load */
+    int tap_loads;        /* 0 = auto (2 with one shot per group and up to 4 x
+                             per thread, else 1), 1 = one tap per coefficient
+                             load, 2 = two taps per 16-byte load */
     int strip_gathers;    /* 0 = auto (on), 1 = on: where every lane of a
                              wavefront sees at most one change of kappa row
                              over its x_per_thread positions, gather each tap
