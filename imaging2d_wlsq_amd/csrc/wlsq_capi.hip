@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -221,7 +222,7 @@ int transpose_tables(Scratch &sc, int nextrap, int nf, int nx, int M, const floa
 struct PlanImpl {
     int M = 0;
     int L = 1;
-    FusedShape shape{4, 1, 1, 256, false, false, false};
+    FusedShape shape{4, 1, 1, 256, false, false, false, false};
     size_t lds = 0;
     int img_lds = 1;
     int img_xpose = 0;
@@ -242,6 +243,7 @@ struct PlanImpl {
     float2 *d_tt = nullptr;  // row-padded table copies: forward at 0, backward at Lp * tt_nk
     int tt_nk = 0;
     int device = 0;
+    std::atomic<int> graphs{0};  // live graphs that captured d_tt: its size is then fixed
     ~PlanImpl()
     {
         if (d_groups)
@@ -255,11 +257,16 @@ struct PlanImpl {
         if (planes.p)
             (void)hipFree(planes.p);
     }
-    // table copy storage for nk samples (allocates: call outside graph capture)
+    // table copy storage for nk samples (allocates: call outside graph capture).
+    // A graph of this plan holds d_tt: it is never reallocated while one lives.
     int reserve_tables(int nk)
     {
         if (nk <= tt_nk)
             return WLSQ_OK;
+        if (graphs.load() > 0)
+            return fail(WLSQ_EINVAL,
+                        "nk=%d above the %d kappa samples a captured graph of this plan holds "
+                        "(destroy the graph or use a new plan)", nk, tt_nk);
         if (d_tt)
             (void)hipFree(d_tt);
         d_tt = nullptr;
@@ -314,6 +321,21 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     for (int i = 1, run = 1; i < nshots; ++i) {
         run = same_shape(shots[i], shots[i - 1]) ? run + 1 : 1;
         best_run = std::max(best_run, run);
+    }
+    // The CPU path's rounding is compiled for one shot per group, 4 x per
+    // thread and 1024 threads, taps summed in order.
+    if (t.rounding < 0 || t.rounding > 1)
+        return fail(WLSQ_EINVAL, "rounding must be 0 or 1 (got %d)", t.rounding);
+    const bool exact = t.rounding == 1;
+    if (exact) {
+        if ((t.x_per_thread != 0 && t.x_per_thread != 4) ||
+            (t.shots_per_group != 0 && t.shots_per_group != 1) ||
+            (t.block != 0 && t.block <= 512))
+            return fail(WLSQ_EINVAL, "rounding=1 needs 4 x per thread, one shot per group and "
+                                     "more than 512 threads");
+        t.x_per_thread = 4;
+        t.shots_per_group = 1;
+        t.tables_symmetric = 0;
     }
     // x per thread and shots per group; the defaults follow an autotuner
     // sweep on MI355X over apertures 512-8192 and 1-16 shots per group
@@ -469,7 +491,8 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // paired tap loads relieve the address unit when no shots share the gathers
     // (1024-thread workgroups too, now that their tap loop is fenced: 4-14 %)
     const bool paired = t.tap_loads == 2 || (t.tap_loads == 0 && S == 1 && R <= 4);
-    P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired};
+    P->shape = FusedShape{R, S, C, block, t.tables_conjugate != 0, t.tables_symmetric != 0, paired,
+                          exact};
     P->img_lds = img_lds;
     // the transpose and the image planes are compiled for the one-shot
     // 1024-thread shapes (the default for apertures whose partial rows do not
@@ -658,7 +681,7 @@ int ktable_device_impl(Device *d, int ns, int nextrap, int nz, int nt, int nf, i
                                t.omegas_per_item, t.block, t.x_per_thread, t.lds_budget,
                                t.tables_conjugate, t.image_mode, t.tables_symmetric, t.tap_loads,
                                t.strip_gathers, t.double_rows, t.depth_steps, t.image_planes,
-                               (long long)(intptr_t)st};  // a plan's table copies are per stream
+                               t.rounding, (long long)(intptr_t)st};  // a plan's table copies are per stream
     std::shared_ptr<PlanImpl> P;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -1076,6 +1099,7 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
         return rc;
     }
     HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->plan->graphs.fetch_add(1);
     *out = g.release();
     return WLSQ_OK;
 }
@@ -1096,6 +1120,8 @@ int wlsq_graph_destroy(wlsq_graph graph)
         (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph)
         (void)hipGraphDestroy(graph->graph);
+    if (graph->exec && graph->plan)
+        graph->plan->graphs.fetch_sub(1);
     delete graph;
     return WLSQ_OK;
 }

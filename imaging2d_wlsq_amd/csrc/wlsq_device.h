@@ -21,6 +21,28 @@ __device__ __forceinline__ float re_mul_conj(float2 a, float2 b)
     return fmaf(a.y, b.y, a.x * b.x);
 }
 
+// acc + a*b with the rounding of the CPU path: the product's parts are
+// rounded separately, (a.x*b.x - a.y*b.y, a.x*b.y + a.y*b.x), then added to
+// acc (thrust::complex<float> products in an unconjugated dot,
+// cpp_src/mkl_linalg.cpp:7-11). No contraction into fused multiply-adds.
+__device__ __forceinline__ float2 cmac_exact(float2 acc, float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    const float re = a.x * b.x - a.y * b.y;
+    const float im = a.x * b.y + a.y * b.x;
+    acc.x = acc.x + re;
+    acc.y = acc.y + im;
+    return acc;
+}
+
+// Re(a * conj(b)) = a.x*b.x + a.y*b.y, each product rounded separately
+// (cpp_src/extrapolation_cpu.cpp:74-79)
+__device__ __forceinline__ float re_mul_conj_exact(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y;
+}
+
 // Nearest kappa sample to omega*s on kappa_q = kappa0 + q/inv_dk, ties to the
 // lower q (numpy argmin in py_src/operatorsPreparation/util.py:5-8). One
 // rounding per operation and no contraction, so the host-side expansion and

@@ -52,9 +52,25 @@ __device__ __forceinline__ void wave_sync()
 // window load is one ds_read_b128 for both fields. Position p of a row lives
 // at sub-row (p mod R), slot p / R (swz): lanes that read p = R*lane + c for a
 // fixed c touch consecutive 16-byte slots.
-template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP,
+          bool EXACT>
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
+    static_assert(!(SYM && EXACT), "the CPU path's rounding sums the taps in order");
+    // complex multiply-add and imaging term: fused multiply-adds, or (EXACT)
+    // the CPU path's separately rounded products
+    auto mac = [](float2 acc, float2 u, float2 v) {
+        if constexpr (EXACT)
+            return cmac_exact(acc, u, v);
+        else
+            return cmac(acc, u, v);
+    };
+    auto rmc = [](float2 u, float2 v) {
+        if constexpr (EXACT)
+            return re_mul_conj_exact(u, v);
+        else
+            return re_mul_conj(u, v);
+    };
     constexpr int LOG2R = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : R == 2 ? 1 : 0;
     static_assert((1 << LOG2R) == R, "R must be a power of two");
     constexpr int M_ = L_ > 0 ? (L_ - 1) / 2 : 0;
@@ -333,8 +349,8 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         for (int s = 0; s < S; ++s) {
 #pragma unroll
                             for (int r = 0; r < R; ++r) {
-                                accF[s][r] = cmac(accF[s][r], cf[r], make_float2(wv4[s][r].x, wv4[s][r].y));
-                                accB[s][r] = cmac(accB[s][r], cb[r], make_float2(wv4[s][r].z, wv4[s][r].w));
+                                accF[s][r] = mac(accF[s][r], cf[r], make_float2(wv4[s][r].x, wv4[s][r].y));
+                                accB[s][r] = mac(accB[s][r], cb[r], make_float2(wv4[s][r].z, wv4[s][r].w));
                             }
                         }
 #pragma unroll
@@ -529,7 +545,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 #pragma unroll
                         for (int s = 0; s < S; ++s)
                             if (s < nshots)
-                                v[r] += re_mul_conj(accF[s][r], accB[s][r]);
+                                v[r] += rmc(accF[s][r], accB[s][r]);
                     }
                     store_row<R>(imgrow + (size_t)w * nxr + x0, v);
                 } else {
@@ -539,7 +555,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                             float v[R];
 #pragma unroll
                             for (int r = 0; r < R; ++r)
-                                v[r] = re_mul_conj(accF[s][r], accB[s][r]);
+                                v[r] = rmc(accF[s][r], accB[s][r]);
                             store_row<R>(imgrow + ((size_t)s * W + w) * nxr + x0, v);
                         }
                     }
@@ -550,7 +566,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 #pragma unroll
                     for (int r = 0; r < R; ++r)
                         if (s < nshots)
-                            xv[s][r] = re_mul_conj(accF[s][r], accB[s][r]);
+                            xv[s][r] = rmc(accF[s][r], accB[s][r]);
                 }
             } else {
 #pragma unroll
@@ -561,7 +577,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                             const int x = x0 + r;
                             if (x < nx)
                                 unsafeAtomicAdd(imgd + ioff[s] + (long long)l * img_ld + x,
-                                                re_mul_conj(accF[s][r], accB[s][r]));
+                                                rmc(accF[s][r], accB[s][r]));
                         }
                     }
                 }
@@ -683,10 +699,11 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     }
 }
 
-template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP>
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP,
+          bool EXACT>
 hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
 {
-    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP>;
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP, EXACT>;
     hipError_t e = ensure_lds(k, lds);
     if (e != hipSuccess)
         return e;
@@ -700,16 +717,39 @@ hipError_t launch_fused_RS(const FusedShape &sh, int n_items, size_t lds, hipStr
 {
     const int b = sh.block;
     if (sh.sym)
-        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT, XP>(n_items, b, lds, st, a)
-                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT, XP>(n_items, b, lds, st, a);
-    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT, XP>(n_items, b, lds, st, a)
-                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT, XP>(n_items, b, lds, st, a);
+        return sh.conj ? launch_fused_t<L_, R, S, true, true, P, C, MAXT, XP, false>(n_items, b, lds, st, a)
+                       : launch_fused_t<L_, R, S, false, true, P, C, MAXT, XP, false>(n_items, b, lds, st, a);
+    return sh.conj ? launch_fused_t<L_, R, S, true, false, P, C, MAXT, XP, false>(n_items, b, lds, st, a)
+                   : launch_fused_t<L_, R, S, false, false, P, C, MAXT, XP, false>(n_items, b, lds, st, a);
+}
+
+// The CPU path's rounding: taps in order (no pairing), compiled for the
+// one-shot 1024-thread shapes only.
+template <int L_, bool P, int C, bool XP>
+hipError_t launch_fused_exact(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
+                              const FusedArgs &a)
+{
+    const int b = sh.block;
+    return sh.conj ? launch_fused_t<L_, 4, 1, true, false, P, C, 1024, XP, true>(n_items, b, lds, st, a)
+                   : launch_fused_t<L_, 4, 1, false, false, P, C, 1024, XP, true>(n_items, b, lds, st, a);
 }
 
 template <int L_, bool P>
 hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStream_t st,
                           const FusedArgs &a)
 {
+    if (sh.exact) {
+        if (!(sh.R == 4 && sh.S == 1 && sh.block > 512))
+            return hipErrorInvalidValue;
+        const bool xp = !a.img_lds && a.img_xpose;
+        if (sh.C == 1)
+            return xp ? launch_fused_exact<L_, P, 1, true>(sh, n_items, lds, st, a)
+                      : launch_fused_exact<L_, P, 1, false>(sh, n_items, lds, st, a);
+        if (sh.C == 2)
+            return xp ? launch_fused_exact<L_, P, 2, true>(sh, n_items, lds, st, a)
+                      : launch_fused_exact<L_, P, 2, false>(sh, n_items, lds, st, a);
+        return hipErrorInvalidValue;
+    }
     if (sh.R == 4 && sh.S == 1 && sh.block > 512) {  // 16 waves: registers for 4 per SIMD
         const bool xp = !a.img_lds && a.img_xpose;
         if (sh.C == 1)

@@ -77,6 +77,7 @@ struct FusedShape {
     bool conj;  // table_back == conj(table_forw): load one table
     bool sym;   // tables are tap-symmetric (T[k] == T[L-1-k]): pair the taps
     bool paired;  // gather two taps per 16-byte load (fewer address-unit cycles)
+    bool exact;   // the CPU path's rounding: separate multiplies and adds, taps in order
 };
 
 // LDS bytes one task needs: two fields x S shots x W rows, swizzled rows of

@@ -28,6 +28,31 @@ def _ptr(t: torch.Tensor, dtype, name: str) -> int:
     return t.data_ptr()
 
 
+def _need(t: torch.Tensor, n: int, name: str) -> None:
+    """The C ABI takes raw pointers and no buffer sizes: a launch whose
+    offsets reach past a tensor would read or write outside it on the device,
+    so every extent is checked here before the call."""
+    if n > t.numel():
+        raise ValueError(f"{name} has {t.numel()} elements, the call reads or writes {n}")
+
+
+def _extent(off: int, rows: int, ld: int, nx: int) -> int:
+    """Elements from the start of a buffer to the end of `rows` rows of `nx`
+    values at stride `ld` from element `off` (0 when there are none)."""
+    if rows <= 0 or nx <= 0:
+        return 0
+    return off + (rows - 1) * ld + nx
+
+
+def _check_uniform(ns, nextrap, nz, nt, nf, nx, forw, back, image):
+    """Fields (ns, nt, nx), rows j < nf used; image (ns, nz, nx), rows l < nextrap."""
+    if ns <= 0 or nx <= 0:
+        return
+    _need(forw, _extent((ns - 1) * nt * nx, nf, nx, nx), "forw")
+    _need(back, _extent((ns - 1) * nt * nx, nf, nx, nx), "back")
+    _need(image, _extent((ns - 1) * nz * nx, nextrap, nx, nx), "image")
+
+
 def _stream(stream) -> int:
     if stream is None:
         return torch.cuda.current_stream().cuda_stream
@@ -49,6 +74,12 @@ def plain_device(ns, nextrap, nz, nt, nf, nx, M, op_forw, forw, op_back, back, i
     expanded (nextrap, nf, [L, nx] | [nx, L]) tables; fields (ns, nt, nx) in
     place; image (ns, nz, nx) rows < nextrap written."""
     c = torch.complex64
+    args = (_ptr(op_forw, c, "op_forw"), _ptr(forw, c, "forw"), _ptr(op_back, c, "op_back"),
+            _ptr(back, c, "back"), _ptr(image, torch.float32, "image"))
+    if min(ns, nextrap, nf, nx) > 0 and M >= 0:
+        _check_uniform(ns, nextrap, nz, nt, nf, nx, forw, back, image)
+        _need(op_forw, nextrap * nf * (2 * M + 1) * nx, "op_forw")
+        _need(op_back, nextrap * nf * (2 * M + 1) * nx, "op_back")
     check(lib().wlsq_plain_device(ns, nextrap, nz, nt, nf, nx, M, _ptr(op_forw, c, "op_forw"),
                                   _ptr(forw, c, "forw"), _ptr(op_back, c, "op_back"),
                                   _ptr(back, c, "back"), _ptr(image, torch.float32, "image"),
@@ -62,10 +93,22 @@ def ktable_device(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, dkapp
     every shot are cleared and then accumulated."""
     c = torch.complex64
     tun = _tuning(tuning)
+    ld = int(slow_ld if slow_ld is not None else nx)
+    for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                        (table_forw, c, "table_forw"), (forw, c, "forw"), (table_back, c, "table_back"),
+                        (back, c, "back"), (image, torch.float32, "image")):
+        _ptr(t, dt, name)
+    if min(ns, nextrap, nx) > 0 and M >= 0:
+        _need(image, _extent((ns - 1) * nz * nx, nextrap, nx, nx), "image")
+        if nf > 0:
+            _check_uniform(ns, nextrap, nz, nt, nf, nx, forw, back, image)
+            _need(slowness, _extent(0, nextrap, ld, nx), "slowness")
+            _need(omega, nf, "omega")
+            _need(table_forw, int(nk) * (2 * M + 1), "table_forw")
+            _need(table_back, int(nk) * (2 * M + 1), "table_back")
     check(lib().wlsq_ktable_device(ns, nextrap, nz, nt, nf, nx, M,
                                    _ptr(slowness, torch.float32, "slowness"),
-                                   int(slow_ld if slow_ld is not None else nx),
-                                   _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
+                                   ld, _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
                                    int(nk), _ptr(table_forw, c, "table_forw"), _ptr(forw, c, "forw"),
                                    _ptr(table_back, c, "table_back"), _ptr(back, c, "back"),
                                    _ptr(image, torch.float32, "image"), int(bool(write_back)),
@@ -75,6 +118,15 @@ def ktable_device(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, dkapp
 def expand_table_device(nextrap, nf, nx, M, slowness, omega, kappa0, dkappa, nk, table, out,
                         layout=LAYOUT_TAPMAJOR, stream=None, slow_ld=None):
     c = torch.complex64
+    ld = int(slow_ld if slow_ld is not None else nx)
+    for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                        (table, c, "table"), (out, c, "out")):
+        _ptr(t, dt, name)
+    if min(nextrap, nf, nx) > 0 and M >= 0:
+        _need(slowness, _extent(0, nextrap, ld, nx), "slowness")
+        _need(omega, nf, "omega")
+        _need(table, int(nk) * (2 * M + 1), "table")
+        _need(out, nextrap * nf * nx * (2 * M + 1), "out")
     check(lib().wlsq_expand_table_device(nextrap, nf, nx, M, _ptr(slowness, torch.float32, "slowness"),
                                          int(slow_ld if slow_ld is not None else nx),
                                          _ptr(omega, torch.float32, "omega"), float(kappa0),
@@ -127,6 +179,26 @@ class ShotSpec:
     omega_off: int
 
 
+def check_shot_extents(shots: Sequence[ShotSpec], M: int, nk: int, slowness, omega, table_forw,
+                       table_back, forw, back, image) -> None:
+    """Every shot's rows lie inside the tensors handed to the batched launch
+    (raises ValueError before any device call)."""
+    L = 2 * M + 1
+    for i, s in enumerate(shots):
+        if s.nx <= 0 or s.nz <= 0 or s.nf <= 0:
+            continue
+        for t, n, name in ((forw, _extent(s.f_off, s.nf, s.ld, s.nx), "forw"),
+                           (back, _extent(s.b_off, s.nf, s.ld, s.nx), "back"),
+                           (image, _extent(s.img_off, s.nz, s.img_ld, s.nx), "image"),
+                           (slowness, _extent(s.slow_off, s.nz, s.slow_ld, s.nx), "slowness"),
+                           (omega, s.omega_off + s.nf, "omega")):
+            if n > t.numel():
+                raise ValueError(f"shot {i}: {name} has {t.numel()} elements, the shot reaches {n}")
+    if any(s.nx > 0 and s.nz > 0 and s.nf > 0 for s in shots):
+        _need(table_forw, int(nk) * L, "table_forw")
+        _need(table_back, int(nk) * L, "table_back")
+
+
 def _shots_array(shots: Sequence[ShotSpec]):
     arr = (Shot * max(1, len(shots)))()
     for i, s in enumerate(shots):
@@ -156,6 +228,12 @@ class Plan:
     def launch(self, slowness, omega, kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
                write_back=False, stream=None):
         c = torch.complex64
+        for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                            (table_forw, c, "table_forw"), (table_back, c, "table_back"),
+                            (forw, c, "forw"), (back, c, "back"), (image, torch.float32, "image")):
+            _ptr(t, dt, name)
+        check_shot_extents(self.shots, self.M, nk, slowness, omega, table_forw, table_back, forw,
+                           back, image)
         check(lib().wlsq_plan_launch(self._h, _ptr(slowness, torch.float32, "slowness"),
                                      _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
                                      int(nk), _ptr(table_forw, c, "table_forw"),
@@ -187,6 +265,14 @@ class Graph:
                  forw, back, image, write_back=False, zero_count=0):
         c = torch.complex64
         self._h = c_void_p()
+        for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                            (table_forw, c, "table_forw"), (table_back, c, "table_back"),
+                            (forw, c, "forw"), (back, c, "back"), (image, torch.float32, "image")):
+            _ptr(t, dt, name)
+        check_shot_extents(plan.shots, plan.M, nk, slowness, omega, table_forw, table_back, forw,
+                           back, image)
+        if zero_count > image.numel():
+            raise ValueError(f"zero_count {zero_count} above the image's {image.numel()} elements")
         self._keep = (plan, slowness, omega, table_forw, table_back, forw, back, image)
         check(lib().wlsq_graph_create(byref(self._h), plan._h,
                                       _ptr(slowness, torch.float32, "slowness"),

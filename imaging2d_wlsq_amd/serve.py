@@ -44,6 +44,7 @@ class Job:
     seed: int
     fmin: float = 2.0
     fmax: float = 50.0
+    rounding: int = 0  # 1: the CPU path's rounding (wlsq_tuning.rounding), bitwise fields
 
     @property
     def points(self) -> int:
@@ -159,6 +160,8 @@ class GpuLauncher:
                                  img_ld=job.nx, slow_ld=p.model.nx, nx=job.nx, nz=job.nz,
                                  nf=job.nf, omega_off=0) for o in offs]
         tun = batch.default_tuning(job.nx, job.nshots > 1, sv.conj, sv.sym, job.nshots)
+        if job.rounding:
+            tun["rounding"] = 1
         plan = device.Plan(specs, t.M, tun)
         plan.launch(sv.slowness, omega, t.kappa0, t.dkappa, t.nk, sv.tf, sv.tb, F, B, image,
                     stream=stream)

@@ -162,6 +162,19 @@ typedef struct {
                              launch stream and summed into the image after the
                              launch, so tasks running side by side do not
                              contend for the same image addresses. */
+    int rounding;         /* 0 = fast: complex products formed with fused
+                             multiply-adds, taps paired where the tables are
+                             symmetric; 1 = the CPU path's rounding: every
+                             complex product formed with separate multiplies
+                             and adds, taps summed in order k = 0..2M from zero
+                             (cpp_src/extrapolation_cpu.cpp:52-53 with
+                             cpp_src/mkl_linalg.cpp:10), imaging term
+                             re*re + im*im. The final fields are then bit for
+                             bit those of the CPU path on the same tables;
+                             about half the speed. Needs 4 x per thread, one
+                             shot per group and more than 512 threads (the
+                             planner picks them when those fields are 0);
+                             tables_symmetric is ignored. */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

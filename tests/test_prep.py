@@ -1,0 +1,212 @@
+"""Operator preparation (imaging2d_wlsq_amd/prep.py) pinned without the
+reference's Python (it imports numba, which this image lacks):
+
+* defWLSQIndices: the reference's own known answers
+  (unittests/test_py_src_util.py:60-111);
+* the wavenumber set: odd nx is fftfreq(nx)*nx*dkx (problemConfig.py:45-46);
+  even nx drops the unpaired Nyquist wavenumber (the only set the reference's
+  design accepts: preparation.py:67,110 make A L x (2N+1), N = (nx-1)//2);
+* the WLSQ operators (preparation.py:64-97) against numpy.linalg.lstsq, an
+  independent least-squares solver, and the normal equations
+  Gamma^H (Gamma t - p) = 0 of every kappa row, for the symmetric operator and
+  every edge operator on its support;
+* the identities the kernels rely on: backward table = conj(forward table)
+  and tap symmetry of the symmetric operator, bit for bit after rounding;
+* the mirror relation between left and right edge operators.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from imaging2d_wlsq_amd import prep
+
+import oracle
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _oracle():
+    oracle.build()
+
+
+# --------------------------------------------------------- defWLSQIndices KATs
+def test_indices_all_symmetric():
+    """test_py_src_util.py:60-71"""
+    assert (prep.wlsq_indices(nx=10, M=4, flip=False, sym=True, extent=1) == 4).all()
+    assert (prep.wlsq_indices(nx=131, M=14, flip=True, sym=True, extent=5)[:15] == 14).all()
+
+
+def test_indices_flip():
+    """test_py_src_util.py:73-79"""
+    f = prep.wlsq_indices(nx=11, M=5, flip=True, sym=False, extent=5)
+    u = prep.wlsq_indices(nx=11, M=5, flip=False, sym=False, extent=5)
+    assert np.array_equal(np.flip(u), f)
+
+
+def test_indices_asymmetric_by_value():
+    """test_py_src_util.py:81-111"""
+    g = prep.wlsq_indices(nx=15, M=5, flip=False, sym=False, extent=4)
+    assert list(g[:4]) == [0, 1, 2, 3]
+    assert (g[4:11] == 5).all()
+    assert list(g[11:]) == [7, 8, 9, 10]
+    g = prep.wlsq_indices(nx=151, M=5, flip=False, sym=False, extent=5)
+    assert list(g[:5]) == [0, 1, 2, 3, 4]
+    assert (g[5:146] == 5).all()
+    assert list(g[147:]) == [7, 8, 9, 10]
+
+
+# ----------------------------------------------------------------- wavenumbers
+@pytest.mark.parametrize("nx", [1, 7, 31, 151, 1001])
+def test_wavenumbers_odd_nx_are_the_fft_set(nx):
+    dx = 13.25
+    dkx = 2 * math.pi / (nx * dx)
+    ref = np.sort(np.fft.fftfreq(nx) * nx * dkx)
+    np.testing.assert_allclose(prep.wavenumbers(nx, dx), ref, rtol=0, atol=1e-12 * dkx * nx)
+
+
+@pytest.mark.parametrize("nx", [2, 8, 32, 256, 1024])
+def test_wavenumbers_even_nx_drop_the_nyquist(nx):
+    dx = 10.0
+    dkx = 2 * math.pi / (nx * dx)
+    kx = prep.wavenumbers(nx, dx)
+    fft = np.sort(np.fft.fftfreq(nx) * nx * dkx)
+    assert len(kx) == nx - 1
+    assert np.isclose(fft[0], -(nx // 2) * dkx)           # the Nyquist entry of the FFT set
+    np.testing.assert_allclose(kx, fft[1:], rtol=0, atol=1e-12 * dkx * nx)
+    np.testing.assert_array_equal(kx, -kx[::-1])           # symmetric set
+
+
+# ------------------------------------------------------------ WLSQ operators
+def _kappa(nx, dx, n=96):
+    """kappa samples from 0 past the largest |kx| (both propagating and
+    evanescent rows)."""
+    kmax = 1.3 * math.pi / dx
+    return np.linspace(0.0, kmax, n)
+
+
+CASES = [(31, 5, 10.0, 10.0), (32, 5, 10.0, 10.0), (151, 12, 13.2, 13.2), (256, 12, 10.0, 7.5),
+         (1024, 15, 10.0, 10.0), (513, 25, 6.25, 6.25)]
+
+
+@pytest.mark.parametrize("nx,M,dx,dz", CASES)
+def test_symmetric_tables_are_the_least_squares_fit(nx, M, dx, dz):
+    """Each forward row T[q] (complex64) is the least-squares solution of
+    Gamma t = P[q] (lstsq, FP64), and the FP64 operator satisfies the normal
+    equations Gamma^H (Gamma t - P[q]) = 0 for every kappa row."""
+    L = 2 * M + 1
+    kap = _kappa(nx, dx)
+    tf, tb = prep.compact_tables(kap, nx, dx, dz, M)
+    kx = prep.wavenumbers(nx, dx)
+    gamma = np.exp(1j * np.outer(kx, (np.arange(L) - M) * dx))
+    P = prep.phase_shift(kap, kx, dz, True)
+    sol = np.linalg.lstsq(gamma, P.T, rcond=None)[0].T
+    scale = np.abs(sol).max()
+    assert np.abs(tf - sol).max() <= 2e-7 * scale          # complex64 rounding of the FP64 fit
+    A = prep.wlsq_operator(kx, M, dx)
+    t64 = P @ A.T
+    resid = gamma.conj().T @ (gamma @ t64.T - P.T)          # (L, nk)
+    bound = np.abs(gamma).sum(0).max() * np.abs(P).max()
+    assert np.abs(resid).max() <= 1e-12 * bound
+    # the fit is not improved by moving any tap (it is the minimum)
+    rng = np.random.default_rng(nx)
+    q = int(rng.integers(len(kap)))
+    base = np.linalg.norm(gamma @ t64[q] - P[q])
+    for _ in range(5):
+        d = 1e-3 * (rng.standard_normal(L) + 1j * rng.standard_normal(L))
+        assert np.linalg.norm(gamma @ (t64[q] + d) - P[q]) >= base
+
+
+@pytest.mark.parametrize("nx,M,dx,dz", CASES)
+def test_backward_is_conjugate_and_taps_symmetric(nx, M, dx, dz):
+    """Wb = conj(Wf): the backward table fitted on its own (lstsq of
+    exp(+i kz dz)) equals the conjugate of the forward one to FP64 rounding,
+    and the stored backward table is that conjugate bit for bit; the stored
+    taps are symmetric bit for bit (odd and even nx)."""
+    L = 2 * M + 1
+    kap = _kappa(nx, dx)
+    tf, tb = prep.compact_tables(kap, nx, dx, dz, M)
+    np.testing.assert_array_equal(tb.view(np.uint32), np.conj(tf).view(np.uint32))
+    rev = np.ascontiguousarray(tf[:, ::-1])
+    np.testing.assert_array_equal(tf.view(np.uint32), rev.view(np.uint32))
+    kx = prep.wavenumbers(nx, dx)
+    gamma = np.exp(1j * np.outer(kx, (np.arange(L) - M) * dx))
+    Pb = prep.phase_shift(kap, kx, dz, False)
+    solb = np.linalg.lstsq(gamma, Pb.T, rcond=None)[0].T
+    Pf = prep.phase_shift(kap, kx, dz, True)
+    solf = np.linalg.lstsq(gamma, Pf.T, rcond=None)[0].T
+    assert np.abs(solb - np.conj(solf)).max() <= 1e-10 * np.abs(solf).max()
+
+
+def test_propagating_operator_is_the_phase_shift_for_small_kx():
+    """Sanity of the sign convention: a plane wave of small kx propagated by
+    the forward row of kappa >> kx picks up exp(-i kz dz) (preparation.py:17-21)."""
+    nx, M, dx, dz = 257, 15, 10.0, 10.0
+    kap = np.array([0.5 * math.pi / dx])
+    tf, _ = prep.compact_tables(kap, nx, dx, dz, M)
+    kx = prep.wavenumbers(nx, dx)
+    n = len(kx) // 2 + 3                                    # a small positive kx
+    x = (np.arange(2 * M + 1) - M) * dx
+    got = np.sum(tf[0] * np.exp(1j * kx[n] * x))
+    want = np.exp(-1j * math.sqrt(kap[0] ** 2 - kx[n] ** 2) * dz)
+    assert abs(got - want) < 1e-2          # the fit's error at one wavenumber
+    assert abs(got - np.conj(want)) > 1.0  # the opposite sign is far off
+
+
+# ---------------------------------------------------------- edge operators
+@pytest.mark.parametrize("nx,M,dx", [(31, 5, 10.0), (32, 5, 10.0), (151, 12, 13.2), (256, 15, 10.0)])
+def test_edge_operators_fit_on_their_support(nx, M, dx):
+    """createWLSQoperators (preparation.py:64-97): operator g < M has no taps
+    m < M-g (they would reach past the left edge), g > M none at
+    m >= 3M+1-g; on its support it is the least-squares operator of the
+    support's columns (lstsq); g = M is the symmetric operator."""
+    L = 2 * M + 1
+    kx = prep.wavenumbers(nx, dx)
+    ops = prep.wlsq_operators(kx, M, dx)
+    np.testing.assert_allclose(ops[M], prep.wlsq_operator(kx, M, dx), rtol=0, atol=1e-12)
+    kap = _kappa(nx, dx, 24)
+    P = prep.phase_shift(kap, kx, 10.0, True)
+    full = np.exp(1j * np.outer(kx, (np.arange(L) - M) * dx))
+    m = np.arange(L)
+    for g in range(L):
+        sup = m >= M - g if g < M else (m < 3 * M + 1 - g if g > M else np.ones(L, bool))
+        assert sup.sum() == L - abs(g - M)
+        assert np.abs(ops[g][~sup]).max(initial=0.0) <= 1e-14
+        t = P @ ops[g].T
+        sol = np.linalg.lstsq(full[:, sup], P.T, rcond=None)[0].T
+        np.testing.assert_allclose(t[:, sup], sol, rtol=0, atol=1e-9 * np.abs(sol).max())
+
+
+@pytest.mark.parametrize("nx,M", [(31, 5), (64, 7), (151, 12)])
+def test_edge_tables_mirror_and_conjugate(nx, M):
+    """Right operator 2M-g is the left operator g mirrored, T[2M-g][q][m] =
+    T[g][q][2M-m]; every backward table is the conjugate forward one."""
+    dx = 10.0
+    kap = _kappa(nx, dx, 40)
+    tf, tb = prep.compact_tables_all(kap, nx, dx, 10.0, M)
+    L = 2 * M + 1
+    scale = np.abs(tf).max()
+    for g in range(M):
+        np.testing.assert_allclose(tf[2 * M - g], tf[g][:, ::-1], rtol=0, atol=3e-7 * scale)
+    np.testing.assert_allclose(tb, np.conj(tf), rtol=0, atol=3e-7 * scale)
+    assert tf.shape == (L, len(kap), L)
+
+
+def test_expand_with_operator_indices_picks_each_points_operator():
+    """expand_table with gidx: point (l, j, i) holds T[gidx[i]][q(l, j, i)],
+    q from the oracle's kappa lookup (preparation.py:109-110)."""
+    rng = np.random.default_rng(3)
+    nextrap, nf, nx, M, nk = 3, 4, 23, 4, 50
+    L = 2 * M + 1
+    T = (rng.standard_normal((L, nk, L)) + 1j * rng.standard_normal((L, nk, L))).astype(np.complex64)
+    slow = (1.0 / rng.uniform(1500, 3000, (nextrap, nx))).astype(np.float32)
+    omega = np.linspace(10, 200, nf).astype(np.float32)
+    k0 = float(np.float32(omega.min() * slow.min()))
+    dk = float(np.float32((omega.max() * slow.max() - k0) / (nk - 1)))
+    g = prep.wlsq_indices(nx, M, sym=False, extent=M)
+    W = prep.expand_table(T, slow, omega, k0, dk, nk, nextrap, gidx=g)
+    assert W.shape == (nextrap, nf, nx, L)
+    for l in range(nextrap):
+        for j in range(nf):
+            for i in range(nx):
+                q = oracle.kappa_index(omega[j], slow[l, i], k0, dk, nk)
+                assert np.array_equal(W[l, j, i], T[g[i], q])
