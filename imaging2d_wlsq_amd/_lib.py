@@ -26,7 +26,7 @@ EXPORTS = [
     "extrapolate_ktable", "wlsq_ktable_device", "wlsq_expand_table_device", "wlsq_batch_device",
     "wlsq_plan_create", "wlsq_plan_info", "wlsq_plan_launch", "wlsq_plan_destroy",
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
-    "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device",
+    "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device", "wlsq_ktable_device_noalloc",
 ]
 
 
@@ -82,6 +82,8 @@ def lib():
     L.wlsq_ktable_device.restype = ci
     L.wlsq_ktable_device.argtypes = [ci] * 7 + [vp, ci, vp, cf, cf, ci, vp, vp, vp, vp, vp, ci,
                                                 POINTER(Tuning), vp]
+    L.wlsq_ktable_device_noalloc.restype = ci
+    L.wlsq_ktable_device_noalloc.argtypes = L.wlsq_ktable_device.argtypes
     L.wlsq_expand_table_device.restype = ci
     L.wlsq_expand_table_device.argtypes = [ci] * 4 + [vp, ci, vp, cf, cf, ci, vp, vp, ci, vp]
     L.wlsq_batch_device.restype = ci

@@ -88,7 +88,9 @@ struct PlanImpl;
 struct Device {
     int ordinal = 0;
     hipDeviceProp_t props{};
-    hipStream_t own_stream = nullptr;  // default priority, used by the host entries
+    hipStream_t own_stream = nullptr;  // default priority, used by the host entries (created
+                                       // by their first call)
+    void *scratch_ptr = nullptr;       // this device's copy of `scratch` (allocation-free entry)
     std::mutex mu;           // scratch map, plan cache, device-pointer entries
     std::mutex host_call_mu; // one host-array entry call at a time (shared scratch)
     std::map<hipStream_t, std::unique_ptr<Scratch>> scratch;
@@ -108,10 +110,19 @@ int current_device(Device **out)
         auto d = std::make_unique<Device>();
         d->ordinal = dev;
         HIP_TRY(hipGetDeviceProperties(&d->props, dev));
-        HIP_TRY(hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
         it = g_devices.emplace(dev, std::move(d)).first;
     }
     *out = it->second.get();
+    return WLSQ_OK;
+}
+
+// The stream of the host-array entries (default priority).
+int host_stream(Device *d, hipStream_t *out)
+{
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (!d->own_stream)
+        HIP_TRY(hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
+    *out = d->own_stream;
     return WLSQ_OK;
 }
 
@@ -286,8 +297,11 @@ bool same_shape(const wlsq_shot &a, const wlsq_shot &b)
            a.img_ld == b.img_ld;
 }
 
+// The work list and launch shape of a batch; with `upload` the group and
+// task descriptors are copied to plan-owned device memory (the allocation-free
+// entry places them itself).
 int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_tuning *tun,
-               std::shared_ptr<PlanImpl> &out)
+               std::shared_ptr<PlanImpl> &out, bool upload = true)
 {
     if (nshots < 0 || (nshots > 0 && !shots))
         return fail(WLSQ_EINVAL, "shots array is NULL");
@@ -546,18 +560,49 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         }
         P->nplanes = K >= 2 ? K : 1;
     }
-    if (!P->groups.empty()) {
+    if (upload && !P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
                           hipMemcpyHostToDevice));
     }
-    if (!P->items.empty()) {
+    if (upload && !P->items.empty()) {
         HIP_TRY(hipMalloc(&P->d_items, sizeof(Item) * P->items.size()));
         HIP_TRY(hipMemcpy(P->d_items, P->items.data(), sizeof(Item) * P->items.size(),
                           hipMemcpyHostToDevice));
     }
     out = P;
     return WLSQ_OK;
+}
+
+// Kernel arguments of a plan's launch over every depth step, image planes off.
+FusedArgs fused_args(const PlanImpl &P, const Item *items, const Group *groups, const float2 *ttF,
+                     const float2 *ttB, const float *slowness, const float *omega, float kappa0,
+                     float dkappa, int nk, wlsq_c32 *forw, wlsq_c32 *back, float *image,
+                     int write_back)
+{
+    FusedArgs a{};
+    a.items = items;
+    a.groups = groups;
+    a.F = reinterpret_cast<float2 *>(forw);
+    a.B = reinterpret_cast<float2 *>(back);
+    a.img = image;
+    a.omega = omega;
+    a.slow = slowness;
+    a.TF = ttF;
+    a.TB = P.shape.conj ? ttF : ttB;
+    a.nk = nk;
+    a.kappa0 = kappa0;
+    a.inv_dk = inv_dkappa(dkappa);
+    a.L = P.L;
+    a.write_back = write_back;
+    a.img_lds = P.img_lds;
+    a.img_xpose = P.img_xpose;
+    a.nplanes = 1;
+    a.strip_gathers = P.strip_gathers;
+    a.double_rows = P.double_rows;
+    a.l0 = 0;
+    a.l1 = P.max_nz;
+    return a;
 }
 
 int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float kappa0,
@@ -576,34 +621,15 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tf), ttF, nk, P.L, st));
     if (!P.shape.conj)
         HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tb), ttB, nk, P.L, st));
-    FusedArgs a{};
-    a.items = P.d_items;
-    a.groups = P.d_groups;
-    a.F = reinterpret_cast<float2 *>(forw);
-    a.B = reinterpret_cast<float2 *>(back);
-    a.img = image;
-    a.omega = omega;
-    a.slow = slowness;
-    a.TF = ttF;
-    a.TB = P.shape.conj ? ttF : ttB;
-    a.nk = nk;
-    a.kappa0 = kappa0;
-    a.inv_dk = inv_dkappa(dkappa);
-    a.L = P.L;
-    a.write_back = write_back;
-    a.img_lds = P.img_lds;
-    a.img_xpose = P.img_xpose;
+    FusedArgs a = fused_args(P, P.d_items, P.d_groups, ttF, ttB, slowness, omega, kappa0, dkappa,
+                             nk, forw, back, image, write_back);
     if (P.nplanes > 1) {
         WLSQ_TRY(P.planes.reserve(sizeof(float) * (size_t)P.nplanes * (size_t)P.img_extent));
         a.planes = P.planes.as<float>();
         a.plane_stride = P.img_extent;
         a.nplanes = P.nplanes;
         HIP_TRY(launch_fill_f32(a.planes, (long long)P.nplanes * P.img_extent, 0.f, st));
-    } else {
-        a.nplanes = 1;
     }
-    a.strip_gathers = P.strip_gathers;
-    a.double_rows = P.double_rows;
     const int ds = P.depth_steps;
     if (ds == 0 || ds >= P.max_nz) {
         a.l0 = 0;
@@ -727,6 +753,35 @@ bool host_tables_conjugate(const wlsq_c32 *tf, const wlsq_c32 *tb, long long n)
 
 }  // namespace
 
+// ------------------------------------------------------- allocation-free entry
+// Device memory of wlsq_ktable_device_noalloc: the plan's group and task
+// descriptors and the row-padded table copies. Each GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress gives for the current device.
+constexpr size_t kNoAllocScratchBytes = 16ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// This device's copy of `scratch`, looked up on the first call per device.
+int device_scratch(Device *d, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(d->mu);
+    if (!d->scratch_ptr) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(WLSQ_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        d->scratch_ptr = ptr;
+    }
+    *out = static_cast<unsigned char *>(d->scratch_ptr);
+    return WLSQ_OK;
+}
+
+}  // namespace
+
 struct wlsq_plan_s {
     std::shared_ptr<PlanImpl> impl;
 };
@@ -771,7 +826,8 @@ int extrapolate(int ns, int nextrap, int nz, int nt, int nf, int nx, int M, wlsq
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
     std::lock_guard<std::mutex> call_lk(d->host_call_mu);
-    hipStream_t st = d->own_stream;
+    hipStream_t st = nullptr;
+    WLSQ_TRY(host_stream(d, &st));
     Scratch *scp;
     {
         std::lock_guard<std::mutex> lk(d->mu);
@@ -884,6 +940,75 @@ int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
                               write_back, tuning, static_cast<hipStream_t>(stream));
 }
 
+int wlsq_ktable_device_noalloc(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                               const float *slowness, int slow_ld, const float *omega,
+                               float kappa0, float dkappa, int nk, const wlsq_c32 *table_forw,
+                               wlsq_c32 *forw_pulse, const wlsq_c32 *table_back,
+                               wlsq_c32 *back_pulse, float *image, int write_back,
+                               const wlsq_tuning *tuning, void *stream)
+{
+    WLSQ_TRY(check_common(ns, nextrap, nz, nt, nf, nx, M));
+    if (ns == 0 || nextrap == 0 || nx == 0)
+        return WLSQ_OK;
+    if (slow_ld < nx)
+        return fail(WLSQ_EINVAL, "slow_ld (%d) < nx (%d)", slow_ld, nx);
+    if (!image || (nf > 0 && (!slowness || !omega || !table_forw || !forw_pulse || !back_pulse)))
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    wlsq_tuning t{};
+    if (tuning)
+        t = *tuning;
+    if (nf > 0 && !table_back && !t.tables_conjugate)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    if (nf > 0 && (nk <= 0 || !(dkappa > 0.0f)))
+        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    if (t.depth_steps != 0)
+        return fail(WLSQ_EINVAL, "the allocation-free entry runs every depth step in one launch "
+                                 "(depth_steps must be 0, got %d)", t.depth_steps);
+    t.image_planes = 1;  // direct image atomics go straight to the image
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::shared_ptr<PlanImpl> P;
+    if (nf > 0) {
+        auto shots = uniform_shots(ns, nextrap, nz, nt, nf, nx, slow_ld);
+        WLSQ_TRY(build_plan(d, shots.data(), ns, M, &t, P, /*upload=*/false));
+    }
+    // scratch layout: groups | tasks | forward table copy | backward table copy
+    const size_t g_bytes = P ? sizeof(Group) * P->groups.size() : 0;
+    const size_t i_bytes = P ? sizeof(Item) * P->items.size() : 0;
+    const size_t t_bytes = P ? sizeof(float2) * (size_t)(P->L + 1) * (size_t)nk : 0;
+    const size_t off_items = align256(g_bytes);
+    const size_t off_tf = off_items + align256(i_bytes);
+    const size_t off_tb = off_tf + align256(t_bytes);
+    const size_t need = off_tb + (P && !P->shape.conj ? t_bytes : 0);
+    if (need > kNoAllocScratchBytes)
+        return fail(WLSQ_ENOMEM,
+                    "the allocation-free entry needs %zu bytes of scratch (table copies %zu, "
+                    "descriptors %zu), above its maximum of %zu", need,
+                    (P && !P->shape.conj ? 2 : 1) * t_bytes, g_bytes + i_bytes,
+                    kNoAllocScratchBytes);
+    unsigned char *base = nullptr;
+    WLSQ_TRY(device_scratch(d, &base));
+    // image rows l < nextrap are outputs: clear them, then accumulate
+    for (int s = 0; s < ns; ++s)
+        HIP_TRY(launch_fill_f32(image + (long long)s * nz * nx, (long long)nextrap * nx, 0.f, st));
+    if (!P || P->items.empty())
+        return WLSQ_OK;
+    Group *groups = reinterpret_cast<Group *>(base);
+    Item *items = reinterpret_cast<Item *>(base + off_items);
+    float2 *ttF = reinterpret_cast<float2 *>(base + off_tf);
+    float2 *ttB = reinterpret_cast<float2 *>(base + off_tb);
+    HIP_TRY(launch_upload(groups, P->groups.data(), g_bytes, st));
+    HIP_TRY(launch_upload(items, P->items.data(), i_bytes, st));
+    HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(table_forw), ttF, nk, P->L, st));
+    if (!P->shape.conj)
+        HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(table_back), ttB, nk, P->L, st));
+    const FusedArgs a = fused_args(*P, items, groups, ttF, ttB, slowness, omega, kappa0, dkappa,
+                                   nk, forw_pulse, back_pulse, image, write_back);
+    HIP_TRY(launch_fused(P->shape, (int)P->items.size(), P->lds, st, a));
+    return WLSQ_OK;
+}
+
 int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
                        const float *slowness, const float *omega, float kappa0, float dkappa,
                        int nk, const wlsq_c32 *table_forw, wlsq_c32 *forw_pulse,
@@ -898,7 +1023,8 @@ int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
         return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
-    hipStream_t st = d->own_stream;
+    hipStream_t st = nullptr;
+    WLSQ_TRY(host_stream(d, &st));
     const int L = 2 * M + 1;
     const long long plane = (long long)nf * nx;
     std::lock_guard<std::mutex> call_lk(d->host_call_mu);

@@ -15,6 +15,9 @@
 //    and gather width.
 #include "wlsq_device.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace wlsq {
 namespace {
 
@@ -214,6 +217,20 @@ __global__ void table_pad_kernel(const float2 *__restrict__ in, float2 *__restri
     out[e] = k < L ? in[q * L + k] : make_float2(0.f, 0.f);
 }
 
+// Host bytes carried to the device in a kernel argument (no host-to-device
+// copy, so nothing is allocated, staged or captured as a copy node).
+constexpr int kUploadWords = 992;  // 3968 bytes: the argument stays under 4 KB
+struct UploadChunk {
+    unsigned nwords;
+    unsigned w[kUploadWords];
+};
+
+__global__ void upload_kernel(unsigned *__restrict__ dst, const UploadChunk c)
+{
+    for (unsigned e = threadIdx.x; e < c.nwords; e += blockDim.x)
+        dst[e] = c.w[e];
+}
+
 __global__ void fill_f32_kernel(float *p, long long n, float v)
 {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -398,6 +415,25 @@ hipError_t launch_planes_reduce(const float *planes, long long stride, int nplan
     hipLaunchKernelGGL(planes_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, st, planes,
                        stride, nplanes, img, n);
     return hipGetLastError();
+}
+
+hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes % 4 != 0 || reinterpret_cast<uintptr_t>(dst) % 4 != 0)
+        return hipErrorInvalidValue;
+    const unsigned *w = static_cast<const unsigned *>(src);
+    unsigned *d = static_cast<unsigned *>(dst);
+    size_t n = bytes / 4;
+    for (size_t o = 0; o < n; o += kUploadWords) {
+        UploadChunk c;
+        c.nwords = (unsigned)std::min<size_t>(kUploadWords, n - o);
+        std::memcpy(c.w, w + o, sizeof(unsigned) * c.nwords);
+        hipLaunchKernelGGL(upload_kernel, dim3(1), dim3(256), 0, st, d + o, c);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st)

@@ -88,9 +88,11 @@ def plain_device(ns, nextrap, nz, nt, nf, nx, M, op_forw, forw, op_back, back, i
 
 def ktable_device(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, dkappa, nk,
                   table_forw, forw, table_back, back, image, write_back=True, tuning=None,
-                  stream=None, slow_ld=None):
+                  stream=None, slow_ld=None, no_alloc=False):
     """Fused compact-table path on device tensors. image rows < nextrap of
-    every shot are cleared and then accumulated."""
+    every shot are cleared and then accumulated. ``no_alloc`` calls the
+    allocation-free entry (library-owned 16 MiB scratch per GPU; calls that
+    overlap on one GPU race)."""
     c = torch.complex64
     tun = _tuning(tuning)
     ld = int(slow_ld if slow_ld is not None else nx)
@@ -106,7 +108,8 @@ def ktable_device(ns, nextrap, nz, nt, nf, nx, M, slowness, omega, kappa0, dkapp
             _need(omega, nf, "omega")
             _need(table_forw, int(nk) * (2 * M + 1), "table_forw")
             _need(table_back, int(nk) * (2 * M + 1), "table_back")
-    check(lib().wlsq_ktable_device(ns, nextrap, nz, nt, nf, nx, M,
+    fn = lib().wlsq_ktable_device_noalloc if no_alloc else lib().wlsq_ktable_device
+    check(fn(ns, nextrap, nz, nt, nf, nx, M,
                                    _ptr(slowness, torch.float32, "slowness"),
                                    ld, _ptr(omega, torch.float32, "omega"), float(kappa0), float(dkappa),
                                    int(nk), _ptr(table_forw, c, "table_forw"), _ptr(forw, c, "forw"),

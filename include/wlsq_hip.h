@@ -185,6 +185,29 @@ int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
                        int write_back, const wlsq_tuning *tuning, void *stream);
 
 /*
+ * The computation of wlsq_ktable_device without any device allocation. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture - the launches it queues are all
+ * kernels, descriptors travel in kernel arguments -, tight serving loops,
+ * harnesses that forbid hipMalloc). The plan's group and task descriptors and
+ * the row-padded table copies live in a library-owned device array of
+ * 16 MiB per GPU, held until the process ends: the call needs
+ * (2M+2)*nk*8 bytes per table read (one when tuning->tables_conjugate) plus
+ * 144 bytes per shot group and 16 per task, and returns WLSQ_ENOMEM with the
+ * sizes above that maximum. tuning->depth_steps must be 0; image planes are
+ * not used (direct image atomics add into `image`). Every call on one GPU
+ * uses that GPU's one array: calls that overlap on one GPU (other streams,
+ * other threads) race and give wrong results; calls queued on one stream
+ * run in order. Same arguments and results as wlsq_ktable_device.
+ */
+int wlsq_ktable_device_noalloc(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
+                               const float *slowness, int slow_ld, const float *omega,
+                               float kappa0, float dkappa, int nk, const wlsq_c32 *table_forw,
+                               wlsq_c32 *forw_pulse, const wlsq_c32 *table_back,
+                               wlsq_c32 *back_pulse, float *image, int write_back,
+                               const wlsq_tuning *tuning, void *stream);
+
+/*
  * Expansion of a compact table into the reference-shaped table
  * (replaces makeTableOfOperators, preparation.py:101-115, for the symmetric
  * operator of every x, main.py:66): out[l][j][i][:] = table[q(l,j,i)][:] in
