@@ -12,7 +12,12 @@ imaging condition, i.e. 16 * 512 * 1024 * 256 = 2.147e9 points.
   (2 x 33.3 GB) resident in HBM, one launch sequence per shot (one launch per
   depth step per shot), fixed-order image reduction.
 * cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference CPU
-  path, compact-table mode) on a bounded sample on this host.
+  path, compact-table mode) on a bounded sample on this host, the 128 x 256
+  case with the reference's expanded tables, and the error of the GPU and CPU
+  paths against an FP64 re-run on stated shot subsets.
+* c3_batched, c5_survey, serving: the uneven 128-shot batch (hipGraph), one
+  GPU's 125-shot share of the 2000 x 8192 survey, and the 2000-job serving
+  stream (p50/p99 latency).
 
 Inputs are resident in HBM before the timed region. Prints ONE JSON line.
 """
@@ -63,17 +68,96 @@ def read_profile_traffic(name):
     return None
 
 
-def cpu_baseline(p, nshots_sample=16):
-    """CPU oracle (port of the reference CPU path) on `nshots_sample` shots of
-    the workload (all 16: about 7 s on 16 cores), all depth steps, every
-    frequency."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _fp64_subset(p, shot, tunings, log, label):
+    """One shot through every depth step on the GPU (each tuning), the CPU
+    oracle and the FP64 re-run on the same inputs: image and final-field
+    relative L2 of each pair."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from oracle import fp64
+    from imaging2d_wlsq_amd import batch
+
+    m, t = p.model, p.tables
+    sv = batch.Survey(p)
+    res = {}
+    Fi = Bi = None
+    dev = {}
+    for name, tun in tunings.items():
+        base = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym))
+        b = sv.batch([shot], tuning=dict(base, **tun), write_back=True)
+        if Fi is None:
+            Fi = b.F.cpu().numpy().reshape(shot.nf, shot.nx)
+            Bi = b.B.cpu().numpy().reshape(shot.nf, shot.nx)
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        dev[name] = (img.cpu().numpy()[:shot.nz, shot.x0:shot.x0 + shot.nx],
+                     b.F.cpu().numpy().reshape(shot.nf, shot.nx), b.B.cpu().numpy().reshape(shot.nf, shot.nx))
+        b.close()
+    sl = np.ascontiguousarray(m.slowness[:shot.nz, shot.x0:shot.x0 + shot.nx])
+    Fo, Bo = Fi[None].copy(), Bi[None].copy()
+    io = np.zeros((1, shot.nz, shot.nx), np.float32)
+    oracle.extrapolate_ktable(1, shot.nz, shot.nz, shot.nx, shot.nf, shot.nf, t.M, sl, shot.omega,
+                              t.kappa0, t.dkappa, t.nk, t.forw, t.back, Fo, Bo, io)
+    t0 = time.perf_counter()
+    kt = dict(slowness=sl, omega=shot.omega, kappa0=t.kappa0, dkappa=t.dkappa, nk=t.nk,
+              forw=t.forw, back=t.back)
+    i64, F64, B64 = fp64.extrapolate(shot.nz, shot.nz, shot.nx, shot.nf, t.M, wave_fs=Fi,
+                                     shot_fs=Bi, ktable=kt)
+    t64 = time.perf_counter() - t0
+
+    def rel(a, b):
+        a = np.asarray(a, np.complex128)
+        b = np.asarray(b, np.complex128)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    fo = np.concatenate([Fo[0].ravel(), Bo[0].ravel()])
+    f64 = np.concatenate([F64.ravel(), B64.ravel()])
+    res["cpu_oracle_vs_fp64"] = {"image": rel(io[0], i64), "fields": rel(fo, f64)}
+    for name, (img, Fd, Bd) in dev.items():
+        fd = np.concatenate([Fd.ravel(), Bd.ravel()])
+        res[f"gpu_{name}_vs_fp64"] = {"image": rel(img, i64), "fields": rel(fd, f64)}
+        res[f"gpu_{name}_vs_cpu_oracle"] = {"image": rel(img, io[0]), "fields": rel(fd, fo),
+                                            "fields_bitwise": bool(np.array_equal(
+                                                fd.view(np.uint64), fo.view(np.uint64)))}
+    res["fp64_seconds"] = t64
+    log(f"{label}: " + ", ".join(f"{k} {v['image']:.2e}" for k, v in res.items() if isinstance(v, dict)))
+    return res
+
+
+def cpu_baseline(p, log, nshots_sample=16):
+    """The CPU leg: the oracle (port of the reference CPU path) timed on this
+    host, and the accuracy of both paths against the FP64 re-run.
+
+    * value: compact-table mode on `nshots_sample` shots of the workload (all
+      16: about 7 s on 16 cores), all depth steps, every frequency;
+    * c1_expanded: the 128 x 256 single-shot case with the expanded
+      (nextrap, nf, nx, L) table, as the reference CPU path reads it
+      (cpp_src/extrapolation_cpu.cpp:39-89);
+    * fp64_errors: image and final-field relative L2 of the GPU (fast path,
+      and the CPU path's rounding where stated), the CPU oracle and an FP64
+      re-run (py_src/wavefieldPropagation/extrapolation.py:38-66) on shot
+      subsets of the workload (one shot, 16 of its 256 frequencies, all 512
+      steps) and of the survey (one full-aperture shot, 2 frequencies, all
+      2000 steps)."""
     import numpy as np
 
     cores = len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(cores)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     import oracle
-    from imaging2d_wlsq_amd import synth
+    from imaging2d_wlsq_amd import prep, synth
 
     oracle.lib()
     t, m = p.tables, p.model
@@ -86,17 +170,46 @@ def cpu_baseline(p, nshots_sample=16):
                               t.kappa0, t.dkappa, t.nk, t.forw, t.back, F, B, img)
     dt = time.perf_counter() - t0
     pts = len(shots) * m.nz * m.nx * s.nf
-    cpu = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": pts / dt, "unit": "points/s", "cores": cores, "kind": "port",
-            "sample": f"{len(shots)} of {len(p.shots)} shots, all {m.nz} depth steps, nf {s.nf}, "
-                      f"nx {m.nx} ({pts:.3g} points, {dt:.1f} s, compact-table mode, {cpu})"}
+    cpu = _cpu_model()
+    out = {"value": pts / dt, "unit": "points/s", "cores": cores, "kind": "port",
+           "sample": f"{len(shots)} of {len(p.shots)} shots, all {m.nz} depth steps, nf {s.nf}, "
+                     f"nx {m.nx} ({pts:.3g} points, {dt:.1f} s, compact-table mode, {cpu})"}
+    log(f"cpu (compact tables, 512x1024): {out['value']:.3e} points/s on {cores} cores")
+
+    c1 = synth.config_c1()
+    s1 = c1.shots[0]
+    t1, m1 = c1.tables, c1.model
+    Wf = prep.expand_table(t1.forw, m1.slowness, s1.omega, t1.kappa0, t1.dkappa, t1.nk, m1.nz)
+    Wb = prep.expand_table(t1.back, m1.slowness, s1.omega, t1.kappa0, t1.dkappa, t1.nk, m1.nz)
+    F1, B1 = synth.host_fields(c1, c1.shots)
+    im1 = np.zeros((1, m1.nz, m1.nx), np.float32)
+    reps = []
+    for _ in range(3):
+        Fa, Ba = F1.copy(), B1.copy()
+        t0 = time.perf_counter()
+        oracle.extrapolate(1, m1.nz, m1.nz, m1.nx, s1.nf, s1.nf, t1.M, Wf, Wb, Fa, Ba, im1)
+        reps.append(time.perf_counter() - t0)
+    p1 = m1.nz * m1.nx * s1.nf
+    out["c1_expanded"] = {"value": p1 / min(reps), "unit": "points/s", "cores": cores,
+                          "seconds": min(reps), "points": p1,
+                          "table_bytes_per_direction": int(Wf.nbytes),
+                          "what": "128 x 256, nf 64, 1 shot, 25 taps: expanded (nextrap, nf, nx, L) "
+                                  "tables as the reference CPU path reads them; best of 3"}
+    log(f"cpu (expanded tables, 128x256): {out['c1_expanded']['value']:.3e} points/s")
+    del Wf, Wb
+
+    errs = {}
+    s2 = p.shots[3]
+    sub = synth.ShotGeom(0, m.nx, m.nz, s2.xs, s2.freqs[::16], s2.seed)
+    errs["c2_shot3_16freqs_512steps"] = _fp64_subset(
+        p, sub, {"fast": {}, "cpu_rounding": dict(rounding=1)}, log, "fp64 errors, 512x1024 subset")
+    p5 = synth.config_c5(nshots=4)
+    s5 = p5.shots[2]
+    sub5 = synth.ShotGeom(0, p5.model.nx, p5.model.nz, s5.xs, s5.freqs[[128, 768]], s5.seed)
+    errs["c5_shot_2freqs_2000steps"] = _fp64_subset(
+        p5, sub5, {"fast": {}, "cpu_rounding": dict(rounding=1)}, log, "fp64 errors, survey subset")
+    out["fp64_errors"] = errs
+    return out
 
 
 def onchip_coefficients(M, L, conj, sym, shots_per_group, points, kernel_ms):
@@ -161,9 +274,13 @@ def bench_c5(args, log):
     p = synth.config_c5()
     sv = batch.Survey(p)
     shots = p.shots[:: max(1, len(p.shots) // args.c5_shots)][:args.c5_shots]
+    warm = sv.batch(shots[:1])  # the same kernel on one shot (first-launch costs)
+    warm.run()
+    torch.cuda.synchronize()
+    warm.close()
+    del warm
     b = sv.batch(shots)
     setup = time.perf_counter() - t0
-    b.run()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -245,11 +362,12 @@ def main():
                     help="main leg: autotune the launch shape (default: the planner's shape, which "
                          "the committed profile covers)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
-    ap.add_argument("--serve-jobs", type=int, default=200,
-                    help="jobs of the serving run (0: skip; the full configuration is 2000)")
+    ap.add_argument("--serve-jobs", type=int, default=2000,
+                    help="jobs of the serving run (0: skip; the configuration is 2000)")
     ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--c5-shots", type=int, default=8,
-                    help="shots of the 2000 x 8192 survey measured (0: skip; the survey has 1000)")
+    ap.add_argument("--c5-shots", type=int, default=125,
+                    help="shots of the 2000 x 8192 survey measured in one launch (0: skip; 125 is "
+                         "one GPU's share of the 1000-shot survey over 8 GPUs)")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="arrival rate as a multiple of the measured saturated service rate")
     ap.add_argument("--tune-verbose", action="store_true")
@@ -449,8 +567,7 @@ def main():
 
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(p)
-        log(out["cpu_baseline"])
+        out["cpu_baseline"] = cpu_baseline(p, log)
     print(json.dumps(out), flush=True)
 
 

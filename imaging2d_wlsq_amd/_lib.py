@@ -27,6 +27,7 @@ EXPORTS = [
     "wlsq_plan_create", "wlsq_plan_info", "wlsq_plan_launch", "wlsq_plan_destroy",
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
     "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device", "wlsq_ktable_device_noalloc",
+    "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws",
 ]
 
 
@@ -89,6 +90,11 @@ def lib():
     L.wlsq_batch_device.restype = ci
     L.wlsq_batch_device.argtypes = [POINTER(Shot), ci, ci, vp, vp, cf, cf, ci, vp, vp, vp, vp, vp,
                                     ci, POINTER(Tuning), vp]
+    L.wlsq_batch_workspace_bytes.restype = c_longlong
+    L.wlsq_batch_workspace_bytes.argtypes = [POINTER(Shot), ci, ci, ci, POINTER(Tuning)]
+    L.wlsq_batch_device_ws.restype = ci
+    L.wlsq_batch_device_ws.argtypes = [POINTER(Shot), ci, ci, vp, vp, cf, cf, ci, vp, vp, vp, vp,
+                                       vp, ci, POINTER(Tuning), vp, c_longlong, vp]
     L.wlsq_plan_create.restype = ci
     L.wlsq_plan_create.argtypes = [POINTER(vp), POINTER(Shot), ci, ci, POINTER(Tuning)]
     L.wlsq_plan_info.restype = ci

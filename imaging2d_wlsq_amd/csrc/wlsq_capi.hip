@@ -753,6 +753,71 @@ bool host_tables_conjugate(const wlsq_c32 *tf, const wlsq_c32 *tb, long long n)
 
 }  // namespace
 
+// ------------------------------------------------- launches into a given workspace
+namespace {
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Layout of a plan's device data in one workspace: groups | tasks | forward
+// table copy | backward table copy (when the tables are not conjugate).
+struct WsLayout {
+    size_t g_bytes = 0, i_bytes = 0, t_bytes = 0;
+    size_t off_items = 0, off_tf = 0, off_tb = 0, need = 0;
+};
+
+WsLayout ws_layout(const PlanImpl &P, int nk)
+{
+    WsLayout w;
+    w.g_bytes = sizeof(Group) * P.groups.size();
+    w.i_bytes = sizeof(Item) * P.items.size();
+    w.t_bytes = sizeof(float2) * (size_t)(P.L + 1) * (size_t)(nk > 0 ? nk : 0);
+    w.off_items = align256(w.g_bytes);
+    w.off_tf = w.off_items + align256(w.i_bytes);
+    w.off_tb = w.off_tf + align256(w.t_bytes);
+    w.need = w.off_tb + (P.shape.conj ? 0 : align256(w.t_bytes));
+    return w;
+}
+
+// The plan's launch over every depth step with its descriptors and table
+// copies placed in `ws` (layout ws_layout): kernels only, nothing allocated.
+int launch_in_workspace(const PlanImpl &P, unsigned char *ws, const WsLayout &w,
+                        const float *slowness, const float *omega, float kappa0, float dkappa,
+                        int nk, const wlsq_c32 *tf, const wlsq_c32 *tb, wlsq_c32 *forw,
+                        wlsq_c32 *back, float *image, int write_back, hipStream_t st)
+{
+    if (P.items.empty())
+        return WLSQ_OK;
+    Group *groups = reinterpret_cast<Group *>(ws);
+    Item *items = reinterpret_cast<Item *>(ws + w.off_items);
+    float2 *ttF = reinterpret_cast<float2 *>(ws + w.off_tf);
+    float2 *ttB = reinterpret_cast<float2 *>(ws + w.off_tb);
+    HIP_TRY(launch_upload(groups, P.groups.data(), w.g_bytes, st));
+    HIP_TRY(launch_upload(items, P.items.data(), w.i_bytes, st));
+    HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tf), ttF, nk, P.L, st));
+    if (!P.shape.conj)
+        HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(tb), ttB, nk, P.L, st));
+    const FusedArgs a = fused_args(P, items, groups, ttF, ttB, slowness, omega, kappa0, dkappa,
+                                   nk, forw, back, image, write_back);
+    HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
+    return WLSQ_OK;
+}
+
+// Tuning of the workspace entries: every depth step in one launch, direct
+// image atomics without planes.
+int workspace_tuning(const wlsq_tuning *tuning, wlsq_tuning *t)
+{
+    *t = wlsq_tuning{};
+    if (tuning)
+        *t = *tuning;
+    if (t->depth_steps != 0)
+        return fail(WLSQ_EINVAL, "launches into a workspace run every depth step in one launch "
+                                 "(depth_steps must be 0, got %d)", t->depth_steps);
+    t->image_planes = 1;
+    return WLSQ_OK;
+}
+
+}  // namespace
+
 // ------------------------------------------------------- allocation-free entry
 // Device memory of wlsq_ktable_device_noalloc: the plan's group and task
 // descriptors and the row-padded table copies. Each GPU the library runs on
@@ -762,8 +827,6 @@ constexpr size_t kNoAllocScratchBytes = 16ull << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
-
-size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 // This device's copy of `scratch`, looked up on the first call per device.
 int device_scratch(Device *d, unsigned char **out)
@@ -952,61 +1015,83 @@ int wlsq_ktable_device_noalloc(int ns, int nextrap, int nz, int nt, int nf, int 
         return WLSQ_OK;
     if (slow_ld < nx)
         return fail(WLSQ_EINVAL, "slow_ld (%d) < nx (%d)", slow_ld, nx);
-    if (!image || (nf > 0 && (!slowness || !omega || !table_forw || !forw_pulse || !back_pulse)))
-        return fail(WLSQ_EINVAL, "NULL device pointer");
     wlsq_tuning t{};
-    if (tuning)
-        t = *tuning;
-    if (nf > 0 && !table_back && !t.tables_conjugate)
+    WLSQ_TRY(workspace_tuning(tuning, &t));
+    if (!image || (nf > 0 && (!slowness || !omega || !table_forw || !forw_pulse || !back_pulse ||
+                              (!table_back && !t.tables_conjugate))))
         return fail(WLSQ_EINVAL, "NULL device pointer");
     if (nf > 0 && (nk <= 0 || !(dkappa > 0.0f)))
         return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
-    if (t.depth_steps != 0)
-        return fail(WLSQ_EINVAL, "the allocation-free entry runs every depth step in one launch "
-                                 "(depth_steps must be 0, got %d)", t.depth_steps);
-    t.image_planes = 1;  // direct image atomics go straight to the image
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
     hipStream_t st = static_cast<hipStream_t>(stream);
     std::shared_ptr<PlanImpl> P;
+    WsLayout w;
     if (nf > 0) {
         auto shots = uniform_shots(ns, nextrap, nz, nt, nf, nx, slow_ld);
         WLSQ_TRY(build_plan(d, shots.data(), ns, M, &t, P, /*upload=*/false));
+        w = ws_layout(*P, nk);
+        if (w.need > kNoAllocScratchBytes)
+            return fail(WLSQ_ENOMEM,
+                        "the allocation-free entry needs %zu bytes of scratch (table copies %zu, "
+                        "descriptors %zu), above its maximum of %zu", w.need,
+                        (P->shape.conj ? 1 : 2) * w.t_bytes, w.g_bytes + w.i_bytes,
+                        kNoAllocScratchBytes);
     }
-    // scratch layout: groups | tasks | forward table copy | backward table copy
-    const size_t g_bytes = P ? sizeof(Group) * P->groups.size() : 0;
-    const size_t i_bytes = P ? sizeof(Item) * P->items.size() : 0;
-    const size_t t_bytes = P ? sizeof(float2) * (size_t)(P->L + 1) * (size_t)nk : 0;
-    const size_t off_items = align256(g_bytes);
-    const size_t off_tf = off_items + align256(i_bytes);
-    const size_t off_tb = off_tf + align256(t_bytes);
-    const size_t need = off_tb + (P && !P->shape.conj ? t_bytes : 0);
-    if (need > kNoAllocScratchBytes)
-        return fail(WLSQ_ENOMEM,
-                    "the allocation-free entry needs %zu bytes of scratch (table copies %zu, "
-                    "descriptors %zu), above its maximum of %zu", need,
-                    (P && !P->shape.conj ? 2 : 1) * t_bytes, g_bytes + i_bytes,
-                    kNoAllocScratchBytes);
     unsigned char *base = nullptr;
     WLSQ_TRY(device_scratch(d, &base));
     // image rows l < nextrap are outputs: clear them, then accumulate
     for (int s = 0; s < ns; ++s)
         HIP_TRY(launch_fill_f32(image + (long long)s * nz * nx, (long long)nextrap * nx, 0.f, st));
-    if (!P || P->items.empty())
+    if (!P)
         return WLSQ_OK;
-    Group *groups = reinterpret_cast<Group *>(base);
-    Item *items = reinterpret_cast<Item *>(base + off_items);
-    float2 *ttF = reinterpret_cast<float2 *>(base + off_tf);
-    float2 *ttB = reinterpret_cast<float2 *>(base + off_tb);
-    HIP_TRY(launch_upload(groups, P->groups.data(), g_bytes, st));
-    HIP_TRY(launch_upload(items, P->items.data(), i_bytes, st));
-    HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(table_forw), ttF, nk, P->L, st));
-    if (!P->shape.conj)
-        HIP_TRY(launch_table_pad(reinterpret_cast<const float2 *>(table_back), ttB, nk, P->L, st));
-    const FusedArgs a = fused_args(*P, items, groups, ttF, ttB, slowness, omega, kappa0, dkappa,
-                                   nk, forw_pulse, back_pulse, image, write_back);
-    HIP_TRY(launch_fused(P->shape, (int)P->items.size(), P->lds, st, a));
-    return WLSQ_OK;
+    return launch_in_workspace(*P, base, w, slowness, omega, kappa0, dkappa, nk, table_forw,
+                               table_back, forw_pulse, back_pulse, image, write_back, st);
+}
+
+long long wlsq_batch_workspace_bytes(const wlsq_shot *shots, int nshots, int M, int nk,
+                                     const wlsq_tuning *tuning)
+{
+    wlsq_tuning t{};
+    if (workspace_tuning(tuning, &t) != WLSQ_OK)
+        return -1;
+    Device *d = nullptr;
+    if (current_device(&d) != WLSQ_OK)
+        return -1;
+    std::shared_ptr<PlanImpl> P;
+    if (build_plan(d, shots, nshots, M, &t, P, /*upload=*/false) != WLSQ_OK)
+        return -1;
+    return (long long)ws_layout(*P, nk).need;
+}
+
+int wlsq_batch_device_ws(const wlsq_shot *shots, int nshots, int M, const float *slowness,
+                         const float *omega, float kappa0, float dkappa, int nk,
+                         const wlsq_c32 *table_forw, const wlsq_c32 *table_back, wlsq_c32 *forw,
+                         wlsq_c32 *back, float *image, int write_back, const wlsq_tuning *tuning,
+                         void *workspace, long long workspace_bytes, void *stream)
+{
+    wlsq_tuning t{};
+    WLSQ_TRY(workspace_tuning(tuning, &t));
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    std::shared_ptr<PlanImpl> P;
+    WLSQ_TRY(build_plan(d, shots, nshots, M, &t, P, /*upload=*/false));
+    if (P->items.empty())
+        return WLSQ_OK;
+    if (!slowness || !omega || !table_forw || (!table_back && !P->shape.conj) || !forw || !back ||
+        !image || !workspace)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    if (nk <= 0 || !(dkappa > 0.0f))
+        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+        return fail(WLSQ_EINVAL, "workspace must be 256-byte aligned");
+    const WsLayout w = ws_layout(*P, nk);
+    if (workspace_bytes < 0 || (size_t)workspace_bytes < w.need)
+        return fail(WLSQ_ENOMEM, "workspace of %lld bytes, the batch needs %zu", workspace_bytes,
+                    w.need);
+    return launch_in_workspace(*P, static_cast<unsigned char *>(workspace), w, slowness, omega,
+                               kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
+                               write_back, static_cast<hipStream_t>(stream));
 }
 
 int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

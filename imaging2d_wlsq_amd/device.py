@@ -210,6 +210,40 @@ def _shots_array(shots: Sequence[ShotSpec]):
     return arr
 
 
+def batch_workspace_bytes(shots: Sequence[ShotSpec], M: int, nk: int,
+                          tuning: Optional[dict] = None) -> int:
+    """Device bytes :func:`batch_launch_ws` needs for this batch."""
+    tun = _tuning(tuning)
+    n = int(lib().wlsq_batch_workspace_bytes(_shots_array(shots), len(shots), M, int(nk),
+                                             byref(tun)))
+    if n < 0:
+        raise ValueError(f"invalid batch: {lib().wlsq_last_error().decode(errors='replace')}")
+    return n
+
+
+def batch_launch_ws(shots: Sequence[ShotSpec], M: int, slowness, omega, kappa0, dkappa, nk,
+                    table_forw, table_back, forw, back, image, workspace: torch.Tensor,
+                    write_back=False, tuning: Optional[dict] = None, stream=None) -> None:
+    """Queue one fused launch of `shots` with its descriptors and table copies
+    in `workspace` (a uint8 device tensor of at least
+    :func:`batch_workspace_bytes`): no device allocation, no wait. Adds into
+    `image`."""
+    c = torch.complex64
+    for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                        (table_forw, c, "table_forw"), (table_back, c, "table_back"),
+                        (forw, c, "forw"), (back, c, "back"), (image, torch.float32, "image"),
+                        (workspace, torch.uint8, "workspace")):
+        _ptr(t, dt, name)
+    check_shot_extents(shots, M, nk, slowness, omega, table_forw, table_back, forw, back, image)
+    tun = _tuning(tuning)
+    check(lib().wlsq_batch_device_ws(_shots_array(shots), len(shots), M, slowness.data_ptr(),
+                                     omega.data_ptr(), float(kappa0), float(dkappa), int(nk),
+                                     table_forw.data_ptr(), table_back.data_ptr(), forw.data_ptr(),
+                                     back.data_ptr(), image.data_ptr(), int(bool(write_back)),
+                                     byref(tun), workspace.data_ptr(), int(workspace.numel()),
+                                     _stream(stream)))
+
+
 class Plan:
     """Work list of a batch of uneven shots for the single fused launch."""
 

@@ -9,7 +9,10 @@ depth count, a frequency list and 1-32 shots. The loop
   budget, first come first served;
 * builds the job's wavefields on its stream and queues ONE fused launch for
   all its shots (shots of a job share window and frequencies, so they are
-  grouped and share coefficient loads);
+  grouped and share coefficient loads). The launch's work list is built on
+  the host and written, with the padded table copies, into the slot's
+  workspace by kernels on the slot's stream (``wlsq_batch_device_ws``): a job
+  makes no device allocation, frees nothing and never waits for the device;
 * polls completion events, frees the job's memory and slot, and records the
   latency arrival -> image ready.
 
@@ -55,11 +58,15 @@ class Job:
         return [synth.ShotGeom(self.x0, self.nx, self.nz, (k + 0.5) * self.nx / self.nshots, freqs,
                                self.seed * 64 + k) for k in range(self.nshots)]
 
-    def footprint(self) -> int:
+    def footprint(self, M: int = 15, nk: int = 4096) -> int:
+        """Device bytes the job holds while it runs: wavefields, image, and its
+        launch workspace (group and task descriptors at one shot per group and
+        one frequency per task, the most there can be, plus the two row-padded
+        table copies of (2M+2) x nk entries)."""
         fields = 2 * self.nshots * self.nf * self.nx * 8
         image = self.nz * self.nx * 4
-        items = math.ceil(self.nshots / 4) * self.nf * 16 + 8 * 128
-        return fields + image + items
+        workspace = self.nshots * (144 + 16 * self.nf) + 2 * (2 * M + 2) * nk * 8 + 1024
+        return fields + image + workspace
 
 
 def make_jobs(n: int, rate: float, n_models: int, model_nz: int, model_nx: int, seed: int = 4,
@@ -124,11 +131,12 @@ class ServeStats:
 
 class GpuLauncher:
     """Runs a job on the GPU: wavefields built on the job's stream, one fused
-    launch for all its shots, a completion event."""
+    launch for all its shots with its descriptors in the slot's workspace, a
+    completion event."""
 
     class Handle:
-        def __init__(self, event, bufs, plan):
-            self.event, self.bufs, self.plan = event, bufs, plan
+        def __init__(self, event, bufs):
+            self.event, self.bufs = event, bufs
 
         def done(self) -> bool:
             return self.event.query()
@@ -140,7 +148,9 @@ class GpuLauncher:
         self.surveys = [batch.Survey(p, dev) for p in models]
         self.dev = torch.device(dev)
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(slots)]
-        self._plans: List[device.Plan] = []  # kept until close(): freeing syncs the device
+        # one grow-only launch workspace per slot: a slot's next job is queued
+        # only after its previous one finished
+        self._ws: List[Optional[torch.Tensor]] = [None] * slots
 
     @property
     def slots(self) -> int:
@@ -162,22 +172,24 @@ class GpuLauncher:
         tun = batch.default_tuning(job.nx, job.nshots > 1, sv.conj, sv.sym, job.nshots)
         if job.rounding:
             tun["rounding"] = 1
-        plan = device.Plan(specs, t.M, tun)
-        plan.launch(sv.slowness, omega, t.kappa0, t.dkappa, t.nk, sv.tf, sv.tb, F, B, image,
-                    stream=stream)
-        self._plans.append(plan)
+        need = device.batch_workspace_bytes(specs, t.M, t.nk, tun)
+        ws = self._ws[slot]
+        if ws is None or ws.numel() < need:
+            with torch.cuda.stream(stream):
+                ws = torch.empty(max(need, 4 << 20), dtype=torch.uint8, device=self.dev)
+            self._ws[slot] = ws
+        device.batch_launch_ws(specs, t.M, sv.slowness, omega, t.kappa0, t.dkappa, t.nk, sv.tf,
+                               sv.tb, F, B, image, ws, tuning=tun, stream=stream)
         ev = torch.cuda.Event()
         ev.record(stream)
-        return GpuLauncher.Handle(ev, (F, B, omega, image), plan)
+        return GpuLauncher.Handle(ev, (F, B, omega, image, ws))
 
     def sync(self):
         torch.cuda.synchronize()
 
     def close(self):
         torch.cuda.synchronize()
-        for p in self._plans:
-            p.close()
-        self._plans.clear()
+        self._ws = [None] * len(self._ws)
 
 
 class Server:

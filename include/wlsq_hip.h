@@ -249,6 +249,23 @@ int wlsq_batch_device(const wlsq_shot *shots, int nshots, int M,
                       wlsq_c32 *forw, wlsq_c32 *back, float *image, int write_back,
                       const wlsq_tuning *tuning, void *stream);
 
+/* The batched form without device allocation and without waiting: the work
+ * list is built on the host, its descriptors and the row-padded table copies
+ * are written into the caller's `workspace` (256-byte aligned, at least
+ * wlsq_batch_workspace_bytes) by kernels queued on `stream`, then the fused
+ * kernel; returns once queued. For loops that launch many batches (the
+ * serving loop keeps one workspace per stream); no counterpart in the
+ * reference. depth_steps must be 0; image planes are not used. A workspace
+ * must not be reused before the launch that wrote it has finished (queue its
+ * next use on the same stream). */
+long long wlsq_batch_workspace_bytes(const wlsq_shot *shots, int nshots, int M, int nk,
+                                     const wlsq_tuning *tuning); /* -1 on invalid input */
+int wlsq_batch_device_ws(const wlsq_shot *shots, int nshots, int M, const float *slowness,
+                         const float *omega, float kappa0, float dkappa, int nk,
+                         const wlsq_c32 *table_forw, const wlsq_c32 *table_back, wlsq_c32 *forw,
+                         wlsq_c32 *back, float *image, int write_back, const wlsq_tuning *tuning,
+                         void *workspace, long long workspace_bytes, void *stream);
+
 /* Reusable plan: the work list (shot groups x frequency groups, longest
  * first) and launch shape are built once and kept on the device. */
 typedef struct wlsq_plan_s *wlsq_plan;

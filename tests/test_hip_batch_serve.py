@@ -382,3 +382,69 @@ def test_graph_keeps_its_table_storage():
         torch.cuda.synchronize()
         assert rel(img.cpu().numpy(), ref.cpu().numpy()) <= 1e-6, rep
     b.close()
+
+
+def test_workspace_launch_matches_plan_launch():
+    """wlsq_batch_device_ws (descriptors and table copies in a caller
+    workspace, nothing allocated) computes what a plan launch computes: final
+    fields bit for bit, the image to the rounding of the atomic order; a
+    workspace below wlsq_batch_workspace_bytes is refused."""
+    import torch
+    from imaging2d_wlsq_amd import batch, device
+    from imaging2d_wlsq_amd._lib import WLSQ_ENOMEM, WlsqError
+    p = _small_problem(seed=16, nz=30, nx=800)
+    shots = _uneven_shots(p, 6, 88)
+    sv = batch.Survey(p)
+    t = p.tables
+    b = sv.batch(shots, write_back=True)
+    F0, B0 = b.F.clone(), b.B.clone()
+    ref = b.run().clone()
+    torch.cuda.synchronize()
+    Fr, Br = b.F.clone(), b.B.clone()
+    tun = dict(b.tuning)
+    need = device.batch_workspace_bytes(b.specs, t.M, t.nk, tun)
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    F, B = F0.clone(), B0.clone()
+    img = torch.zeros_like(ref)
+    device.batch_launch_ws(b.specs, t.M, sv.slowness, b.omega, t.kappa0, t.dkappa, t.nk, sv.tf, sv.tb,
+                           F, B, img, ws, write_back=True, tuning=tun)
+    torch.cuda.synchronize()
+    print("workspace bytes", need, "plan", b.plan.info(), flush=True)
+    assert torch.equal(F, Fr) and torch.equal(B, Br)
+    assert rel(img.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+    small = torch.empty(need - 256, dtype=torch.uint8, device="cuda")
+    with pytest.raises(WlsqError) as e:
+        device.batch_launch_ws(b.specs, t.M, sv.slowness, b.omega, t.kappa0, t.dkappa, t.nk, sv.tf,
+                               sv.tb, F, B, img, small, tuning=tun)
+    assert e.value.status == WLSQ_ENOMEM
+    b.close()
+
+
+def test_serving_device_memory_stays_flat():
+    """A stream of 300 served jobs: the device memory outside torch's caching
+    allocator (what the library itself would hold) does not grow, and every
+    job's tensors are released when it completes."""
+    import torch
+    from imaging2d_wlsq_amd import serve
+    models = [_small_problem(41), _small_problem(42, nz=50, nx=700)]
+    launcher = serve.GpuLauncher(models, slots=4)
+
+    def outside_torch():
+        torch.cuda.synchronize()
+        free, total = torch.cuda.mem_get_info()
+        return total - free - torch.cuda.memory_reserved()
+
+    warm = serve.make_jobs(40, 1e9, 2, 40, 600, seed=21, nx_range=(64, 600), nz_range=(8, 40),
+                           nf_range=(4, 24), shots_range=(1, 6))
+    serve.Server(launcher, hbm_budget=1 << 34).run(warm)
+    base_out, base_alloc = outside_torch(), torch.cuda.memory_allocated()
+    jobs = serve.make_jobs(300, 2000.0, 2, 40, 600, seed=22, nx_range=(64, 600), nz_range=(8, 40),
+                           nf_range=(4, 24), shots_range=(1, 6))
+    st = serve.Server(launcher, hbm_budget=1 << 34).run(jobs)
+    out, alloc = outside_torch(), torch.cuda.memory_allocated()
+    print(f"served {len(st.latencies)} jobs; device memory outside torch {base_out} -> {out}, "
+          f"torch allocated {base_alloc} -> {alloc}", flush=True)
+    assert len(st.latencies) == 300
+    assert out - base_out <= (4 << 20)
+    assert alloc - base_alloc <= (64 << 20)   # at most the slots' grown workspaces
+    launcher.close()

@@ -114,5 +114,9 @@ def test_oversized_job_is_rejected_up_front():
 
 def test_footprint_counts_fields_image_and_work_list():
     j = serve.Job(0, 0, 0, nx=1000, nz=500, nf=100, nshots=3, arrival=0.0, seed=1)
-    assert j.footprint() >= 2 * 3 * 100 * 1000 * 8 + 500 * 1000 * 4
+    fields_image = 2 * 3 * 100 * 1000 * 8 + 500 * 1000 * 4
+    tables = 2 * 32 * 4096 * 8       # the launch's two padded table copies (M 15, nk 4096)
+    descriptors = 3 * 144 + 3 * 100 * 16
+    assert j.footprint() >= fields_image + tables + descriptors
+    assert j.footprint(M=25, nk=8192) >= fields_image + 2 * 52 * 8192 * 8
     assert j.points == 1000 * 500 * 100 * 3
