@@ -425,15 +425,18 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
                 return false;
         return true;
     };
-    // image partial rows in LDS unless some group cannot fit them at W = 1;
-    // then direct atomics, made lane-contiguous through a wave-private LDS
-    // segment in single-frequency tasks (mode 3), or per point as computed
-    // (mode 2)
-    if (t.image_mode < 0 || t.image_mode > 3)
-        return fail(WLSQ_EINVAL, "image_mode must be 0, 1, 2 or 3 (got %d)", t.image_mode);
-    int img_lds = t.image_mode >= 2 ? 0 : 1;
+    // The image sum over a task's frequencies is formed in LDS - from
+    // per-frequency partial rows (mode 1), or straight from the new field rows
+    // after each step's barrier (mode 4: no partial rows, so more frequency
+    // rows fit a task; the default for one shot per group) - unless some
+    // group cannot fit the partial rows at W = 1; then direct atomics, made
+    // lane-contiguous through a wave-private LDS segment in single-frequency
+    // tasks (mode 3), or per point as computed (mode 2).
+    if (t.image_mode < 0 || t.image_mode > 4)
+        return fail(WLSQ_EINVAL, "image_mode must be 0..4 (got %d)", t.image_mode);
+    int img_lds = (t.image_mode == 2 || t.image_mode == 3) ? 0 : 1;
     for (auto &rn : runs) {
-        if (!img_lds)
+        if (!img_lds || t.image_mode == 4)
             break;
         const int rows = run_shared(rn) ? 1 : S;
         if (fused_lds_bytes(shots[rn.first].nx, M, R, S, 1, rows) > lds_max) {
@@ -443,6 +446,10 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
             img_lds = 0;
         }
     }
+    if (t.image_mode == 4 && S != 1)
+        return fail(WLSQ_EUNSUPPORTED, "image_mode 4 is compiled for one shot per group");
+    if (img_lds && (t.image_mode == 4 || (t.image_mode == 0 && S == 1)))
+        img_lds = 2;
     std::vector<Pending> pend;
     size_t lds_need = 0;
     for (auto &rn : runs) {
@@ -465,7 +472,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         g.omega_off = s.omega_off;
         g.img_shared = run_shared(rn) ? 1 : 0;
         g.pad_ = 0;
-        const int img_rows = img_lds ? (g.img_shared ? 1 : S) : 0;
+        const int img_rows = img_lds == 1 ? (g.img_shared ? 1 : S) : 0;
         const int strips = (s.nx + R - 1) / R;
         int W = block * C / strips;
         if (t.omegas_per_item > 0)

@@ -53,9 +53,9 @@ __device__ __forceinline__ int kappa_index(float omega, float s, float kappa0, f
     float t = omega * s;
     t = t - kappa0;
     float u = t * inv_dk;
-    const float hi = (float)(nk - 1);
-    u = (u > 0.0f) ? u : 0.0f;
-    u = (u > hi) ? hi : u;
+    // clamp to [0, nk-1] in one instruction; a NaN u gives q = 0 either way
+    // (the conversion of NaN is 0), as the rule's "not above 0 -> 0" does
+    u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(nk - 1));
     return (int)ceilf(u - 0.5f);
 }
 

@@ -57,6 +57,10 @@ template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAX
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
     static_assert(!(SYM && EXACT), "the CPU path's rounding sums the taps in order");
+    // image sums formed from the new rows (img_lds 2): compiled for one shot
+    // per group (a third branch in the step's imaging code makes the
+    // multi-shot shapes spill)
+    constexpr bool ROWS_IMG = S == 1;
     // complex multiply-add and imaging term: fused multiply-adds, or (EXACT)
     // the CPU path's separately rounded products
     auto mac = [](float2 acc, float2 u, float2 v) {
@@ -531,19 +535,22 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 (void)changes;
                 step(std::integral_constant<int, 0>{});
             }
-            // imaging condition of this step. With img_lds the thread stores
+            // imaging condition of this step. With img_lds 1 the thread stores
             // its R values (summed over the shots when they share an image)
             // in the partial row of its frequency; the flush below sums the
             // task's frequency rows in fixed order, one atomic per (depth, x).
-            if (a.img_lds) {
+            // With img_lds 2 the flush forms the terms itself from the new
+            // rows (nothing to do here).
+            if (ROWS_IMG && a.img_lds == 2) {
+            } else if (a.img_lds) {
                 const int w = wv[cs];
                 if (shared_img) {
                     float v[R];
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        v[r] = 0.f;
+                        v[r] = rmc(accF[0][r], accB[0][r]);  // a group has at least one shot
 #pragma unroll
-                        for (int s = 0; s < S; ++s)
+                        for (int s = 1; s < S; ++s)
                             if (s < nshots)
                                 v[r] += rmc(accF[s][r], accB[s][r]);
                     }
@@ -628,8 +635,12 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         }
         }
         };
+        // position x0 + r + M of a row is at sub-row (r + M) mod R, slot
+        // x0 / R + (r + M) / R (x0 is a multiple of R): the per-r part is the
+        // same for every thread
         auto store_strip = [&](auto cs_c) {
             constexpr int cs = decltype(cs_c)::value;
+            const int sbase = wv[cs] * rowlen + (x0v[cs] >> LOG2R);
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -637,9 +648,39 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                     const int x = x0v[cs] + r;
                     if (x < nx) {
                         const float2 f = accF_[cs][s][r], b = accB_[cs][s][r];
-                        rows_w[s * shot_rows + (size_t)wv[cs] * rowlen + swz<R>(x + M, Q)] =
-                            make_float4(f.x, f.y, b.x, b.y);
+                        const int c = ((r + M) & (R - 1)) * Q + ((r + M) >> LOG2R);
+                        rows_w[s * shot_rows + sbase + c] = make_float4(f.x, f.y, b.x, b.y);
                     }
+                }
+            }
+        };
+        // img_lds 2: Re(F conj B) of the rows written this step, summed per x
+        // over the shots sharing an image (shot order), then over the task's
+        // frequency rows (row order), one atomic per (depth, x) - the order
+        // of the partial rows of img_lds 1
+        auto flush_rows = [&](const float4 *src, int ld_l) {
+            const int nimg = shared_img ? 1 : nshots;
+            for (int si = 0; si < nimg; ++si) {
+                float *dst = a.img + gp->img_off[si] + (long long)ld_l * img_ld;
+                for (int x = tid; x < nx; x += NT) {
+                    const int ps = swz<R>(x + M, Q);
+                    float v = 0.f;
+                    for (int w = 0; w < W; ++w) {
+                        float u;
+                        if (shared_img) {
+                            const float4 e = src[(size_t)w * rowlen + ps];
+                            u = rmc(make_float2(e.x, e.y), make_float2(e.z, e.w));
+                            for (int s = 1; s < nshots; ++s) {
+                                const float4 g = src[((size_t)s * W + w) * rowlen + ps];
+                                u += rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                            }
+                        } else {
+                            const float4 e = src[((size_t)si * W + w) * rowlen + ps];
+                            u = rmc(make_float2(e.x, e.y), make_float2(e.z, e.w));
+                        }
+                        v = w == 0 ? u : v + u;
+                    }
+                    unsafeAtomicAdd(dst + x, v);
                 }
             }
         };
@@ -662,7 +703,12 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 if (actv[1])
                     store_strip(c1{});
         }
-        if (a.img_lds) {
+        // With two copies the rows written this step are complete after the
+        // barrier and stay unchanged until the barrier of the next step.
+        if constexpr (ROWS_IMG)
+            if (a.img_lds == 2 && nbuf == 2)
+                flush_rows(rows_w, l);
+        if (a.img_lds == 1) {
             // consecutive lanes flush consecutive x: one 32-byte sector of the
             // image carries 8 atomic adds
             for (int s = 0; s < img_rows; ++s) {
@@ -676,8 +722,14 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 }
             }
         }
-        if (nbuf == 1)
+        if (nbuf == 1) {
             __syncthreads();
+            // one copy: the new rows are complete after the second barrier;
+            // the next step writes them only after its first barrier
+            if constexpr (ROWS_IMG)
+                if (a.img_lds == 2)
+                    flush_rows(rows, l);
+        }
     }
 
     if (a.write_back) {
@@ -741,7 +793,7 @@ hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStre
     if (sh.exact) {
         if (!(sh.R == 4 && sh.S == 1 && sh.block > 512))
             return hipErrorInvalidValue;
-        const bool xp = !a.img_lds && a.img_xpose;
+        const bool xp = a.img_lds == 0 && a.img_xpose;
         if (sh.C == 1)
             return xp ? launch_fused_exact<L_, P, 1, true>(sh, n_items, lds, st, a)
                       : launch_fused_exact<L_, P, 1, false>(sh, n_items, lds, st, a);
@@ -751,7 +803,7 @@ hipError_t launch_fused_L(const FusedShape &sh, int n_items, size_t lds, hipStre
         return hipErrorInvalidValue;
     }
     if (sh.R == 4 && sh.S == 1 && sh.block > 512) {  // 16 waves: registers for 4 per SIMD
-        const bool xp = !a.img_lds && a.img_xpose;
+        const bool xp = a.img_lds == 0 && a.img_xpose;
         if (sh.C == 1)
             return xp ? launch_fused_RS<L_, P, 4, 1, 1, 1024, true>(sh, n_items, lds, st, a)
                       : launch_fused_RS<L_, P, 4, 1, 1, 1024>(sh, n_items, lds, st, a);

@@ -119,8 +119,9 @@ typedef struct {
     int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
     int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
                              bit for bit, so only table_forw is read */
-    int image_mode;       /* 0 = auto (1 where the partial rows fit next to the
-                             field rows, else 3), 1 = per-frequency image
+    int image_mode;       /* 0 = auto (where the partial rows fit next to the
+                             field rows: 4 with one shot per group, else 1;
+                             elsewhere 3), 1 = per-frequency image
                              partial rows in LDS, summed in fixed order, then one
                              atomic add per (depth, x) per task, 2 = an atomic add
                              per point from the thread that computed it, 3 = as 2,
@@ -130,7 +131,12 @@ typedef struct {
                              wave-private LDS segment (64 x 4 floats per
                              wavefront), so each atomic instruction covers
                              consecutive x, and the tasks add into image planes
-                             (image_planes); other shapes add as in 2 */
+                             (image_planes); other shapes add as in 2;
+                             4 = as 1, but the sum over the task's
+                             frequencies (and shots sharing an image) is
+                             formed from the new field rows after each step,
+                             in the same order, without partial rows: more
+                             frequency rows fit a task (one shot per group) */
     int tables_symmetric; /* 1: caller guarantees both tables are tap-symmetric,
                              T[q][k] == T[q][2M-k] bit for bit (the WLSQ design
                              with symmetric taps gives this); taps are then

@@ -545,7 +545,11 @@ def test_errors_are_reported_not_silent():
         with pytest.raises(_lib.WlsqError):
             device.Plan(specs, b["M"], dict(image_mode=3, image_planes=bad))
     with pytest.raises(_lib.WlsqError):
-        device.Plan(specs, b["M"], dict(image_mode=4))
+        device.Plan(specs, b["M"], dict(image_mode=5))
+    with pytest.raises(_lib.WlsqError):
+        device.Plan(specs, b["M"], dict(rounding=2))
+    with pytest.raises(_lib.WlsqError):  # the CPU path's rounding needs one shot per group
+        device.Plan(specs, b["M"], dict(rounding=1, shots_per_group=2))
     with pytest.raises(_lib.WlsqError):
         device.Plan(specs, b["M"], dict(x_per_thread=8, shots_per_group=2))
     for k in ("nx", "ld", "img_ld", "slow_ld"):

@@ -448,3 +448,46 @@ def test_serving_device_memory_stays_flat():
     assert out - base_out <= (4 << 20)
     assert alloc - base_alloc <= (64 << 20)   # at most the slots' grown workspaces
     launcher.close()
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(shots_per_group=1), dict(shots_per_group=1, double_rows=2),
+    dict(shots_per_group=1, rounding=1), dict(shots_per_group=1, x_per_thread=8),
+    dict(shots_per_group=1, x_per_thread=4, block=1024, omegas_per_item=1)],
+    ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
+@pytest.mark.parametrize("shared", [True, False])
+def test_image_sums_from_rows_match_partial_rows(tuning, shared):
+    """Image mode 4 (the sum over a task's frequencies formed from the new
+    field rows after each step, no partial rows) against mode 1 (partial rows
+    in LDS): the same final fields bit for bit and the same image to the
+    rounding of the atomic order, with one and two copies of the rows, groups
+    sharing one image or each adding its own, and the CPU path's rounding
+    (mode 4 is compiled for one shot per group)."""
+    import torch
+    from imaging2d_wlsq_amd import batch, synth
+    p = _small_problem(seed=17, nz=25, nx=700)
+    base = _uneven_shots(p, 3, 99)
+    shots = [synth.ShotGeom(s.x0, s.nx, s.nz, s.xs + 0.5 * k, s.freqs, s.seed + k)
+             for s in base for k in range(4)]
+    sv = batch.Survey(p)
+    outs = []
+    for mode in (1, 4):
+        t = dict(tuning, image_mode=mode, tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym))
+        b = sv.batch(shots, tuning=t, write_back=True)
+        if not shared:  # every shot adds into its own image (rows of one stacked buffer)
+            m = p.model
+            b.image = torch.zeros((len(shots), m.nz, m.nx), dtype=torch.float32, device="cuda")
+            for i, sp in enumerate(b.specs):
+                sp.img_off = i * m.nz * m.nx + shots[i].x0
+            b.plan.close()
+            from imaging2d_wlsq_amd import device
+            b.plan = device.Plan(b.specs, p.tables.M, t)
+        print("tuning", t, b.plan.info(), flush=True)
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), b.F.cpu().numpy(), b.B.cpu().numpy(), b.plan.info()))
+        b.close()
+    assert outs[1][3]["items"] <= outs[0][3]["items"]   # no fewer frequencies per task
+    np.testing.assert_array_equal(outs[1][1], outs[0][1])
+    np.testing.assert_array_equal(outs[1][2], outs[0][2])
+    assert rel(outs[1][0], outs[0][0]) <= 1e-6
