@@ -141,6 +141,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     float *const imgd = (XP && a.nplanes > 1)
                             ? a.planes + (long long)(blockIdx.x % a.nplanes) * a.plane_stride
                             : a.img;
+    // image sums from the rows (img_lds 2) add into a plane too when the plan has them
+    float *const imgf = (ROWS_IMG && a.nplanes > 1)
+                            ? a.planes + (long long)(blockIdx.x % a.nplanes) * a.plane_stride
+                            : a.img;
     // image partial rows in LDS: one per (image, frequency row); every entry
     // x < nxr is written by its owner thread each step before it is read
     const int img_rows = shared_img ? 1 : nshots;
@@ -661,7 +665,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         auto flush_rows = [&](const float4 *src, int ld_l) {
             const int nimg = shared_img ? 1 : nshots;
             for (int si = 0; si < nimg; ++si) {
-                float *dst = a.img + gp->img_off[si] + (long long)ld_l * img_ld;
+                float *dst = imgf + gp->img_off[si] + (long long)ld_l * img_ld;
                 for (int x = tid; x < nx; x += NT) {
                     const int ps = swz<R>(x + M, Q);
                     float v = 0.f;

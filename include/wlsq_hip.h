@@ -119,8 +119,8 @@ typedef struct {
     int lds_budget;      /* 0 = auto: bytes of LDS per workgroup */
     int tables_conjugate; /* 1: caller guarantees table_back == conj(table_forw)
                              bit for bit, so only table_forw is read */
-    int image_mode;       /* 0 = auto (where the partial rows fit next to the
-                             field rows: 4 with one shot per group, else 1;
+    int image_mode;       /* 0 = auto (4 with one shot per group; else 1 where
+                             the partial rows fit next to the field rows,
                              elsewhere 3), 1 = per-frequency image
                              partial rows in LDS, summed in fixed order, then one
                              atomic add per (depth, x) per task, 2 = an atomic add
@@ -167,7 +167,9 @@ typedef struct {
                              planes of plan-owned device memory, zeroed on the
                              launch stream and summed into the image after the
                              launch, so tasks running side by side do not
-                             contend for the same image addresses. */
+                             contend for the same image addresses. With
+                             image_mode 4, n >= 2 planes are used likewise
+                             (only when asked for). */
     int rounding;         /* 0 = fast: complex products formed with fused
                              multiply-adds, taps paired where the tables are
                              symmetric; 1 = the CPU path's rounding: every

@@ -453,7 +453,8 @@ def test_serving_device_memory_stays_flat():
 @pytest.mark.parametrize("tuning", [
     dict(shots_per_group=1), dict(shots_per_group=1, double_rows=2),
     dict(shots_per_group=1, rounding=1), dict(shots_per_group=1, x_per_thread=8),
-    dict(shots_per_group=1, x_per_thread=4, block=1024, omegas_per_item=1)],
+    dict(shots_per_group=1, x_per_thread=4, block=1024, omegas_per_item=1),
+    dict(shots_per_group=1, x_per_thread=4, block=1024, omegas_per_item=1, image_planes=3)],
     ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 @pytest.mark.parametrize("shared", [True, False])
 def test_image_sums_from_rows_match_partial_rows(tuning, shared):
