@@ -223,6 +223,35 @@ def onchip_coefficients(M, L, conj, sym, shots_per_group, points, kernel_ms):
             "achieved": tbs, "peak": L1_PEAK_TBS, "unit": "TB/s", "frac": tbs / L1_PEAK_TBS}
 
 
+PROBE_BLOCKS, PROBE_ITERS = 8192, 8192
+
+
+def probe_fma(log):
+    """The FMA probe: 8192 workgroups x 4 waves, each wave 8 x 8192 FP32 FMAs
+    in independent chains; HIP-event time of the second of two launches."""
+    import torch
+
+    from imaging2d_wlsq_amd._lib import check, lib
+
+    out = torch.empty(PROBE_BLOCKS * 256, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        check(lib().wlsq_probe_fma(PROBE_BLOCKS, PROBE_ITERS, out.data_ptr(), stream.cuda_stream))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    waves = PROBE_BLOCKS * 4
+    fma = waves * 8 * PROBE_ITERS
+    rate = fma / (ms[-1] / 1e3)
+    log(f"fma probe: {ms[-1]:.3f} ms, {rate:.3e} wave64 FMA instructions/s "
+        f"({rate * 64 * 2 / 1e12:.1f} TFLOP/s)")
+    return {"ms": ms[-1], "wave_fma_instructions": fma, "wave_instructions_per_s": rate,
+            "tflops": rate * 64 * 2 / 1e12}
+
+
 def bench_c3(args, log):
     """128 shots of uneven aperture / frequency count on an 800 x 2048 model in
     one launch, replayed as a hipGraph; launch shape autotuned on 16 shots."""
@@ -373,6 +402,9 @@ def main():
     ap.add_argument("--tune-verbose", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--probe-fma", action="store_true",
+                    help="run the FMA probe of known instruction count (calibrates the VALU "
+                         "counters; its time gives the reachable FMA issue rate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
@@ -414,6 +446,8 @@ def main():
         b.copy_(a)  # 1 GiB read + 1 GiB written
         torch.cuda.synchronize()
         del a, b
+
+    probe = probe_fma(log) if args.probe_fma else None
 
     # ---------------------------------------------------------------- batched
     specs = [device.ShotSpec(f_off=s * nf * nx, b_off=s * nf * nx, img_off=0, slow_off=0, ld=nx,
@@ -494,6 +528,9 @@ def main():
                      "onchip": onchip_coefficients(t.M, L, conj, sym, info["shots_per_group"],
                                                    points, k_avg)},
     }
+
+    if probe is not None:
+        out["probe_fma"] = probe
 
     # ---------------------------------------------------------------- plain
     if not args.no_plain:

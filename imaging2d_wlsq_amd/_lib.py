@@ -27,7 +27,7 @@ EXPORTS = [
     "wlsq_plan_create", "wlsq_plan_info", "wlsq_plan_launch", "wlsq_plan_destroy",
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
     "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device", "wlsq_ktable_device_noalloc",
-    "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws",
+    "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws", "wlsq_probe_fma",
 ]
 
 
@@ -111,6 +111,8 @@ def lib():
     L.wlsq_graph_launch.argtypes = [vp, vp]
     L.wlsq_graph_destroy.restype = ci
     L.wlsq_graph_destroy.argtypes = [vp]
+    L.wlsq_probe_fma.restype = ci
+    L.wlsq_probe_fma.argtypes = [ci, ci, vp, vp]
     L.wlsq_batch_points.restype = c_longlong
     L.wlsq_batch_points.argtypes = [POINTER(Shot), ci]
     # transforms: cpp_src/interface.py:36-45 binds the MKL pair the same way

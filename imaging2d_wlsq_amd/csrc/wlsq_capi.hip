@@ -1197,6 +1197,14 @@ int wlsq_expand_table_device(int nextrap, int nf, int nx, int M, const float *sl
     return WLSQ_OK;
 }
 
+int wlsq_probe_fma(int blocks, int iters, float *out, void *stream)
+{
+    if (!out || blocks <= 0 || iters < 0)
+        return fail(WLSQ_EINVAL, "probe: invalid arguments (blocks=%d iters=%d)", blocks, iters);
+    HIP_TRY(launch_probe_fma(blocks, iters, out, static_cast<hipStream_t>(stream)));
+    return WLSQ_OK;
+}
+
 long long wlsq_batch_points(const wlsq_shot *shots, int nshots)
 {
     long long p = 0;

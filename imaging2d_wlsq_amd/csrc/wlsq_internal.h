@@ -121,6 +121,8 @@ hipError_t launch_expand(const float *slow, int slow_ld, const float *omega, flo
                          float inv_dk, int nk, const float2 *table, float2 *out, int nextrap,
                          int nf, int nx, int L, int layout, hipStream_t st);
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st);
+// blocks x 256 threads, 8 * iters FP32 FMAs per wave (counter calibration)
+hipError_t launch_probe_fma(int blocks, int iters, float *out, hipStream_t st);
 // `bytes` (a multiple of 4) of host memory to `dst` through kernel arguments
 // (chunks under 4 KB): no host-to-device copy
 hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t st);

@@ -231,6 +231,24 @@ __global__ void upload_kernel(unsigned *__restrict__ dst, const UploadChunk c)
         dst[e] = c.w[e];
 }
 
+// Counter calibration: each thread runs `iters` rounds of eight independent
+// FP32 fused multiply-add chains, so every wave issues 8 * iters FMA
+// instructions in its loop, plus a short fixed tail (seven adds and the store).
+__global__ void __launch_bounds__(256) probe_fma_kernel(float *out, float m, float c, int iters)
+{
+    float a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        a[k] = (float)threadIdx.x * 1e-3f + (float)k;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            a[k] = fmaf(a[k], m, c);
+    }
+    out[blockIdx.x * (long long)blockDim.x + threadIdx.x] =
+        ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
 __global__ void fill_f32_kernel(float *p, long long n, float v)
 {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -434,6 +452,14 @@ hipError_t launch_upload(void *dst, const void *src, size_t bytes, hipStream_t s
             return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_probe_fma(int blocks, int iters, float *out, hipStream_t st)
+{
+    if (blocks <= 0 || iters < 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(probe_fma_kernel, dim3(blocks), dim3(256), 0, st, out, 0.999f, 1e-3f, iters);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st)

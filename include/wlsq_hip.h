@@ -305,6 +305,14 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
 int wlsq_graph_launch(wlsq_graph graph, void *stream);
 int wlsq_graph_destroy(wlsq_graph graph);
 
+/* Measurement aid, no counterpart in the reference: `blocks` workgroups of
+ * 256 threads (4 waves) whose every wave issues exactly 8 * iters FP32 FMA
+ * instructions in eight independent chains (plus a 7-add tail and one store
+ * of a float per thread into out[blocks * 256]). Its known instruction count
+ * calibrates the VALU counters of a profile, and its time gives the FMA
+ * issue rate the device reaches with packed FP32 disabled. */
+int wlsq_probe_fma(int blocks, int iters, float *out, void *stream);
+
 /* Points (x, z, omega, shot) the batched form computes for these shots. */
 long long wlsq_batch_points(const wlsq_shot *shots, int nshots);
 
