@@ -1,8 +1,12 @@
 """Migration driver: the reference's ``main.py`` with a ``device_hip`` option.
 
-    python -m imaging2d_wlsq_amd.driver velmod.csv problemSetup.txt shotsDir/ outDir device_hip
+    python -m imaging2d_wlsq_amd.driver velmod.csv problemSetup.txt shotsDir/ outDir device_hip [asym]
 
-Same arguments as ``main.py:30-34``. The steps restate ``main.py``:
+Same arguments as ``main.py:30-34``; an optional sixth argument ``asym``
+selects the edge operators near the model's sides (``defWLSQIndices`` with
+sym=False, extent 5, ``util.py:16-37``) instead of the symmetric operator
+everywhere that ``main.py:66`` asks for (``device_hip_expanded`` only). The
+steps restate ``main.py``:
 
 1. read the velocity CSV, smooth the slowness with a Gaussian of sigma 2 and
    save ``smoothed_velmod.csv`` (``main.py:39-43``);
@@ -132,9 +136,15 @@ def find_shots(shots_dir: str) -> List[Tuple[int, str]]:
 
 
 def migrate(velocity: np.ndarray, setup_text: str, shot_ix: List[int], shots: List[np.ndarray],
-            option: str = "device_hip", log=print):
+            option: str = "device_hip", log=print, sym: bool = True, extent: int = 5):
     """Smoothed velocity and final image (nz, nx) float32 of a shot-profile
-    migration; ``shots`` are (nt, nx) float32 receiver records."""
+    migration; ``shots`` are (nt, nx) float32 receiver records. ``sym=False``
+    uses the edge operators of ``createWLSQoperators`` on the first and last M
+    columns (``defWLSQIndices(nx, M, False, False, extent)``); their per-x
+    choice needs the expanded tables of ``device_hip_expanded``."""
+    if not sym and option != "device_hip_expanded":
+        raise ValueError("edge operators (sym=False) are chosen per x: use device_hip_expanded, "
+                         "whose expanded tables carry a per-point operator")
     t0 = time.time()
     vel = smooth_velocity(np.asarray(velocity, np.float64))
     nz, nx = vel.shape
@@ -166,8 +176,14 @@ def migrate(velocity: np.ndarray, setup_text: str, shot_ix: List[int], shots: Li
             F[s] = spfft.fft(sources[s], axis=0)
             B[s] = spfft.fft(np.asarray(shots[s], np.float32), axis=0)
         image = np.zeros((ns, nz, nx), np.float32)
-        opF = prep.expand_table(tf, slowness, omega, kap0, dk, nk, cfg.nextrap)
-        opB = prep.expand_table(tb, slowness, omega, kap0, dk, nk, cfg.nextrap)
+        if sym:
+            opF = prep.expand_table(tf, slowness, omega, kap0, dk, nk, cfg.nextrap)
+            opB = prep.expand_table(tb, slowness, omega, kap0, dk, nk, cfg.nextrap)
+        else:  # main.py:65-66 with sym=False, makeTableOfOperators (preparation.py:101-115)
+            tfa, tba = prep.compact_tables_all(kappa, nx, cfg.dx, cfg.dz, cfg.M)
+            g = prep.wlsq_indices(nx, cfg.M, flip=False, sym=False, extent=extent)
+            opF = prep.expand_table(tfa, slowness, omega, kap0, dk, nk, cfg.nextrap, gidx=g)
+            opB = prep.expand_table(tba, slowness, omega, kap0, dk, nk, cfg.nextrap, gidx=g)
         interface_hip.extrapolate(ns, cfg.nextrap, nz, cfg.nt, cfg.nw, nx, cfg.M, opF, F, opB, B, image)
     else:
         raise ValueError(f"unknown option {option!r} (device_hip, device_hip_expanded)")
@@ -206,14 +222,16 @@ def _migrate_on_device(cfg: ProblemConfig, nz, nx, slowness, omega, kap0, dk, nk
 
 def main(argv: Optional[List[str]] = None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
-    if len(argv) != 5:
+    if len(argv) not in (5, 6) or (len(argv) == 6 and argv[5] != "asym"):
         print(__doc__)
         return 2
-    file_vel, file_cfg, shots_dir, out_dir, option = argv
+    file_vel, file_cfg, shots_dir, out_dir, option = argv[:5]
+    sym = len(argv) == 5
     velocity = np.genfromtxt(file_vel, delimiter=",")
     found = find_shots(shots_dir)
     shots = [np.genfromtxt(f, delimiter=",", dtype=np.float32) for _, f in found]
-    vel, final = migrate(velocity, open(file_cfg).read(), [ix for ix, _ in found], shots, option)
+    vel, final = migrate(velocity, open(file_cfg).read(), [ix for ix, _ in found], shots, option,
+                         sym=sym)
     os.makedirs(out_dir, exist_ok=True)
     np.savetxt(os.path.join(out_dir, "smoothed_velmod.csv"), vel, delimiter=",")
     np.savetxt(os.path.join(out_dir, "final_image.csv"), final, delimiter=",")

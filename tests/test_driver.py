@@ -86,3 +86,34 @@ def test_demo_migration_matches_reference_image(demo, option):
     print(f"{option}: demo final image rel L2 vs reference {err:.2e}", flush=True)
     assert err <= 5e-3
     assert not final[136:].any()
+
+
+def test_edge_operators_need_the_expanded_tables(demo):
+    with pytest.raises(ValueError):
+        driver.migrate(demo["velocity"], str(demo["setup"]), list(demo["shot_ix"]),
+                       list(demo["shots"]), "device_hip", sym=False)
+
+
+@pytest.mark.gpu
+def test_demo_migration_with_edge_operators(demo):
+    """The demo with the edge operators on the first and last M columns
+    (defWLSQIndices(sym=False, extent=5), util.py:16-37) through the
+    expanded-table drop-in. The demo is 151 wide: for odd nx each edge
+    operator is the symmetric operator with the taps past the edge cut
+    (tests/test_prep.py), and those taps only multiply the zero halo, so the
+    image is that of the symmetric operators to rounding."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    args = (demo["velocity"], str(demo["setup"]), list(demo["shot_ix"]), list(demo["shots"]),
+            "device_hip_expanded")
+    _, sym_img = driver.migrate(*args)
+    _, asym_img = driver.migrate(*args, sym=False)
+    ref = demo["final_image"]
+    diff = float(np.linalg.norm(asym_img - sym_img) / np.linalg.norm(sym_img))
+    err = float(np.linalg.norm(asym_img - ref) / np.linalg.norm(ref))
+    print(f"edge operators on the demo: {diff:.2e} from the symmetric-operator image, {err:.2e} from "
+          f"the reference's image", flush=True)
+    assert diff <= 1e-6
+    assert err <= 5e-3
+    assert not asym_img[136:].any()

@@ -210,3 +210,29 @@ def test_expand_with_operator_indices_picks_each_points_operator():
             for i in range(nx):
                 q = oracle.kappa_index(omega[j], slow[l, i], k0, dk, nk)
                 assert np.array_equal(W[l, j, i], T[g[i], q])
+
+
+@pytest.mark.parametrize("nx,M", [(31, 5), (151, 21), (257, 15), (32, 5), (150, 21), (256, 15)])
+def test_edge_operators_are_the_cut_symmetric_operator_exactly_when_nx_is_odd(nx, M):
+    """For odd nx the kx set is the whole DFT set, so Gamma's columns are
+    orthogonal (Gamma^H Gamma = nK I): each edge operator is the symmetric
+    operator with its outside taps cut, and those taps only ever multiply the
+    zero halo - the edge operators change nothing (the reference's demo,
+    nx = 151). For even nx the unpaired Nyquist wavenumber is dropped, the
+    columns are no longer orthogonal and the edge operators are fits of their
+    own."""
+    dx = 10.0
+    L = 2 * M + 1
+    kx = prep.wavenumbers(nx, dx)
+    ops = prep.wlsq_operators(kx, M, dx)
+    m = np.arange(L)
+    gamma = np.exp(1j * np.outer(kx, (m - M) * dx))
+    gram_off = np.abs(gamma.conj().T @ gamma - len(kx) * np.eye(L)).max()
+    worst = 0.0
+    for g in range(L):
+        sup = m >= M - g if g < M else (m < 3 * M + 1 - g if g > M else np.ones(L, bool))
+        worst = max(worst, np.abs(ops[g][sup] - ops[M][sup]).max() / np.abs(ops[M]).max())
+    if nx % 2:
+        assert gram_off <= 1e-10 * len(kx) and worst <= 1e-12
+    else:
+        assert gram_off > 0.5 and worst > 1e-3
