@@ -134,6 +134,20 @@ Scratch &scratch_for(Device *d, hipStream_t st)
     return *p;
 }
 
+// The kappa grid: nk samples of spacing dkappa. The lookup's rounding is
+// exact up to 2^22 samples, and the fused kernel addresses table rows of L + 1
+// entries with 32-bit byte offsets (L = 0: no table addressed here).
+int check_kappa(int nk, float dkappa, int L)
+{
+    if (nk <= 0 || !(dkappa > 0.0f))
+        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    if (nk > kMaxKappaSamples)
+        return fail(WLSQ_EUNSUPPORTED, "nk=%d above %d kappa samples", nk, kMaxKappaSamples);
+    if (L > 0 && (unsigned long long)nk * (unsigned long long)(L + 1) * 8ull > 0xffffffffull)
+        return fail(WLSQ_EUNSUPPORTED, "table of %d rows of %d taps above 4 GiB", nk, L);
+    return WLSQ_OK;
+}
+
 int check_common(int ns, int nextrap, int nz, int nt, int nf, int nx, int M)
 {
     if (ns < 0 || nextrap < 0 || nz < 0 || nt < 0 || nf < 0 || nx < 0 || M < 0)
@@ -625,8 +639,7 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
         return WLSQ_OK;
     if (!slowness || !omega || !tf || (!tb && !P.shape.conj) || !forw || !back || !image)
         return fail(WLSQ_EINVAL, "NULL device pointer");
-    if (nk <= 0 || !(dkappa > 0.0f))
-        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    WLSQ_TRY(check_kappa(nk, dkappa, P.L));
     WLSQ_TRY(P.reserve_tables(nk));
     float2 *ttF = P.d_tt;
     float2 *ttB = P.d_tt + (size_t)(P.L + 1) * nk;
@@ -1032,8 +1045,8 @@ int wlsq_ktable_device_noalloc(int ns, int nextrap, int nz, int nt, int nf, int 
     if (!image || (nf > 0 && (!slowness || !omega || !table_forw || !forw_pulse || !back_pulse ||
                               (!table_back && !t.tables_conjugate))))
         return fail(WLSQ_EINVAL, "NULL device pointer");
-    if (nf > 0 && (nk <= 0 || !(dkappa > 0.0f)))
-        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    if (nf > 0)
+        WLSQ_TRY(check_kappa(nk, dkappa, 2 * M + 1));
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1093,8 +1106,7 @@ int wlsq_batch_device_ws(const wlsq_shot *shots, int nshots, int M, const float 
     if (!slowness || !omega || !table_forw || (!table_back && !P->shape.conj) || !forw || !back ||
         !image || !workspace)
         return fail(WLSQ_EINVAL, "NULL device pointer");
-    if (nk <= 0 || !(dkappa > 0.0f))
-        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    WLSQ_TRY(check_kappa(nk, dkappa, P->L));
     if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
         return fail(WLSQ_EINVAL, "workspace must be 256-byte aligned");
     const WsLayout w = ws_layout(*P, nk);
@@ -1116,8 +1128,7 @@ int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
         return WLSQ_OK;
     if (!slowness || !omega || !table_forw || !table_back || !forw_pulse || !back_pulse || !image)
         return fail(WLSQ_EINVAL, "NULL array");
-    if (nk <= 0 || !(dkappa > 0.0f))
-        return fail(WLSQ_EINVAL, "invalid kappa grid (nk=%d dkappa=%g)", nk, (double)dkappa);
+    WLSQ_TRY(check_kappa(nk, dkappa, 2 * M + 1));
     Device *d = nullptr;
     WLSQ_TRY(current_device(&d));
     hipStream_t st = nullptr;
@@ -1187,8 +1198,9 @@ int wlsq_expand_table_device(int nextrap, int nf, int nx, int M, const float *sl
         return WLSQ_OK;
     if (nf > 65535 || nextrap > 65535)
         return fail(WLSQ_EUNSUPPORTED, "nf and nextrap must be <= 65535");
-    if (slow_ld < nx || nk <= 0 || !(dkappa > 0.0f))
-        return fail(WLSQ_EINVAL, "invalid slowness stride or kappa grid");
+    if (slow_ld < nx)
+        return fail(WLSQ_EINVAL, "invalid slowness stride %d < nx %d", slow_ld, nx);
+    WLSQ_TRY(check_kappa(nk, dkappa, 0));
     if (!slowness || !omega || !table || !out)
         return fail(WLSQ_EINVAL, "NULL device pointer");
     HIP_TRY(launch_expand(slowness, slow_ld, omega, kappa0, inv_dkappa(dkappa), nk,

@@ -162,6 +162,8 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const float2 *__restrict__ TF = a.TF;
     const float2 *__restrict__ TB = a.TB;
     const int Lp = L + 1;
+    // byte offset of table row q is q * Lp8; the lookup returns -q
+    const int nLp8 = -8 * Lp;
 
     // Each thread owns C strips (task slots tid, tid + NT, ...): slot ->
     // (frequency row w, strip of R consecutive x).
@@ -181,12 +183,18 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 
     float2 accF_[C][S][R], accB_[C][S][R];
     float sv_[C][R];  // one strip per thread: slowness of the coming step at the strip's x
+    unsigned soff[R];  // one strip per thread: byte offsets of those x in a slowness row
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        soff[r] = (unsigned)min(x0v[0] + r, nx - 1) * 4u;
 #pragma unroll
     for (int cs = 0; cs < C; ++cs) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            sv_[cs][r] = (C == 1 && actv[cs]) ? slow0[(long long)lb * slow_ld + min(x0v[cs] + r, nx - 1)] : 0.f;
+            sv_[cs][r] = (C == 1 && actv[cs]) ? at_bytes(slow0 + (long long)lb * slow_ld, soff[r]) : 0.f;
     }
+    // image sums from the rows: the slot of x = tid in a padded row
+    const int fps0 = swz<R>(tid + M, Q);
     for (int l = lb; l < le; ++l) {
         const int cur = nbuf == 2 ? ((l - lb) & 1) : 0;
         const float4 *rows_r = rows + cur * rows_size;               // read this step
@@ -216,22 +224,35 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             // the previous one, and the next step's load is issued now so its
             // latency hides behind the taps. Two strips per thread have no
             // registers to spare and load at the start of the step.
-            int qo[R];
+            // byte offsets of the strip's table rows
+            unsigned qo[R];
             if constexpr (C == 1) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    qo[r] = kappa_index(om, sv_[cs][r], a.kappa0, a.inv_dk, a.nk) * Lp;
-                if (l + 1 < le) {
-                    const float *srow = slow0 + (long long)(l + 1) * slow_ld;
+                    qo[r] = (unsigned)__mul24(kappa_index_neg(om, sv_[cs][r], a.kappa0, a.inv_dk, a.nk), nLp8);
+                // One shot per group: the last step loads its own row again,
+                // with no branch, so the loads go straight into the slowness
+                // registers (groups of shots keep the branch: their register
+                // allocation does better with it).
+                if (S == 1 || l + 1 < le) {
+                    const float *srow = slow0 + (long long)min(l + 1, le - 1) * slow_ld;
 #pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        sv_[cs][r] = srow[min(x0 + r, nx - 1)];
+                    for (int r = 0; r < R; ++r) {
+                        // opaque to loop-invariant code motion: the offset is
+                        // widened here, next to the load, which then takes the
+                        // row as its scalar base (no 64-bit address per lane)
+                        asm volatile("" : "+v"(soff[r]));
+                        sv_[cs][r] = at_bytes(srow, soff[r]);
+                    }
                 }
             } else {
                 const float *srow = slow0 + (long long)l * slow_ld;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    qo[r] = kappa_index(om, srow[min(x0 + r, nx - 1)], a.kappa0, a.inv_dk, a.nk) * Lp;
+                    qo[r] = (unsigned)__mul24(
+                        kappa_index_neg(om, at_bytes(srow, (unsigned)min(x0 + r, nx - 1) * 4u), a.kappa0,
+                                        a.inv_dk, a.nk),
+                        nLp8);
             }
             // Kappa rows are piecewise constant along x. When every active lane
             // of the wavefront has one row over its strip (mode 1), or one
@@ -239,13 +260,6 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             // strip's coefficients come from one or two gathers per tap instead
             // of R (mode 0). Every mode uses the same coefficients.
             bool sel[R];
-            int changes = 0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                sel[r] = qo[r] == qo[0];
-                if (r > 0)
-                    changes += qo[r] != qo[r - 1] ? 1 : 0;
-            }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
 #pragma unroll
@@ -268,35 +282,43 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             auto ldw = [&](int s, int j) { return bs[s][j & (R - 1)][j >> LOG2R]; };
             auto step = [&](auto mode_c) {
                 constexpr int MODE = decltype(mode_c)::value;
-                auto ld4 = [](const float2 *T, int o) { return *reinterpret_cast<const float4 *>(T + o); };
+                // taps k, k+1 of the row at byte offset o
+                auto ld4 = [](const float2 *T, unsigned o, int k) {
+                    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(T) + o + 8 * k);
+                };
+                auto ld2 = [](const float2 *T, unsigned o, int k) {
+                    return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(T) + o + 8 * k);
+                };
                 // coefficients of taps k, k+1 (k even) for every x of the strip
                 auto fetch2 = [&](const int k, float2 (&f0)[R], float2 (&f1)[R], float2 (&b0)[R],
                                   float2 (&b1)[R]) {
                     float4 va{}, vb{}, ua{}, ub{};
                     if constexpr (MODE != 0) {
-                        va = ld4(TF, qo[0] + k);
+                        va = ld4(TF, qo[0], k);
                         if constexpr (!CONJ)
-                            ua = ld4(TB, qo[0] + k);
+                            ua = ld4(TB, qo[0], k);
                         if constexpr (MODE == 2) {
-                            vb = ld4(TF, qo[R - 1] + k);
+                            vb = ld4(TF, qo[R - 1], k);
                             if constexpr (!CONJ)
-                                ub = ld4(TB, qo[R - 1] + k);
+                                ub = ld4(TB, qo[R - 1], k);
                         }
                     }
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         float4 v, u{};
                         if constexpr (MODE == 0) {
-                            v = ld4(TF, qo[r] + k);
+                            v = ld4(TF, qo[r], k);
                             if constexpr (!CONJ)
-                                u = ld4(TB, qo[r] + k);
+                                u = ld4(TB, qo[r], k);
                         } else if constexpr (MODE == 1) {
                             v = va;
                             u = ua;
                         } else {
-                            v = sel[r] ? va : vb;
+                            // x0 is on row qo[0] and x0 + R - 1 on row qo[R - 1]
+                            const bool lo = r == 0 || (r != R - 1 && sel[r]);
+                            v = lo ? va : vb;
                             if constexpr (!CONJ)
-                                u = sel[r] ? ua : ub;
+                                u = lo ? ua : ub;
                         }
                         f0[r] = make_float2(v.x, v.y);
                         f1[r] = make_float2(v.z, v.w);
@@ -313,23 +335,24 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 auto fetch1 = [&](const int k, float2 (&f0)[R], float2 (&b0)[R]) {
                     float2 va{}, vb{}, ua{}, ub{};
                     if constexpr (MODE != 0) {
-                        va = TF[qo[0] + k];
+                        va = ld2(TF, qo[0], k);
                         if constexpr (!CONJ)
-                            ua = TB[qo[0] + k];
+                            ua = ld2(TB, qo[0], k);
                         if constexpr (MODE == 2) {
-                            vb = TF[qo[R - 1] + k];
+                            vb = ld2(TF, qo[R - 1], k);
                             if constexpr (!CONJ)
-                                ub = TB[qo[R - 1] + k];
+                                ub = ld2(TB, qo[R - 1], k);
                         }
                     }
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         if constexpr (MODE == 0) {
-                            f0[r] = TF[qo[r] + k];
-                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : TB[qo[r] + k];
+                            f0[r] = ld2(TF, qo[r], k);
+                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : ld2(TB, qo[r], k);
                         } else {
-                            f0[r] = (MODE == 1 || sel[r]) ? va : vb;
-                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : ((MODE == 1 || sel[r]) ? ua : ub);
+                            const bool lo = MODE == 1 || r == 0 || (r != R - 1 && sel[r]);
+                            f0[r] = lo ? va : vb;
+                            b0[r] = CONJ ? make_float2(f0[r].x, -f0[r].y) : (lo ? ua : ub);
                         }
                     }
                 };
@@ -529,14 +552,29 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             // several shots already share each gather over the shots and are
             // VALU-bound: they keep one gather per x and its register budget.
             if constexpr (S == 1) {
-                if (a.strip_gathers && __ballot(changes != 0) == 0)
+                // mode 1: one row over the strip (lowest row == highest row)
+                unsigned qlo = qo[0], qhi = qo[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    qlo = min(qlo, qo[r]);
+                    qhi = max(qhi, qo[r]);
+                }
+                if (a.strip_gathers && __ballot(qlo != qhi) == 0) {
                     step(std::integral_constant<int, 1>{});
-                else if (a.strip_gathers && __ballot(changes > 1) == 0)
-                    step(std::integral_constant<int, 2>{});
-                else
-                    step(std::integral_constant<int, 0>{});
+                } else {
+                    int changes = 0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        sel[r] = qo[r] == qo[0];
+                        if (r > 0)
+                            changes += qo[r] != qo[r - 1] ? 1 : 0;
+                    }
+                    if (a.strip_gathers && __ballot(changes > 1) == 0)
+                        step(std::integral_constant<int, 2>{});
+                    else
+                        step(std::integral_constant<int, 0>{});
+                }
             } else {
-                (void)changes;
                 step(std::integral_constant<int, 0>{});
             }
             // imaging condition of this step. With img_lds 1 the thread stores
@@ -658,34 +696,31 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                 }
             }
         };
-        // img_lds 2: Re(F conj B) of the rows written this step, summed per x
-        // over the shots sharing an image (shot order), then over the task's
-        // frequency rows (row order), one atomic per (depth, x) - the order
-        // of the partial rows of img_lds 1
+        // img_lds 2 (one shot per group): Re(F conj B) of the rows written
+        // this step, summed per x over the task's frequency rows in row order,
+        // one atomic per (depth, x). Thread t takes x = t, t + NT, ...; the
+        // slot of x + NT is that of x plus NT / R (NT is a multiple of R).
         auto flush_rows = [&](const float4 *src, int ld_l) {
-            const int nimg = shared_img ? 1 : nshots;
-            for (int si = 0; si < nimg; ++si) {
-                float *dst = imgf + gp->img_off[si] + (long long)ld_l * img_ld;
-                for (int x = tid; x < nx; x += NT) {
-                    const int ps = swz<R>(x + M, Q);
-                    float v = 0.f;
-                    for (int w = 0; w < W; ++w) {
-                        float u;
-                        if (shared_img) {
-                            const float4 e = src[(size_t)w * rowlen + ps];
-                            u = rmc(make_float2(e.x, e.y), make_float2(e.z, e.w));
-                            for (int s = 1; s < nshots; ++s) {
-                                const float4 g = src[((size_t)s * W + w) * rowlen + ps];
-                                u += rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
-                            }
-                        } else {
-                            const float4 e = src[((size_t)si * W + w) * rowlen + ps];
-                            u = rmc(make_float2(e.x, e.y), make_float2(e.z, e.w));
-                        }
-                        v = w == 0 ? u : v + u;
-                    }
-                    unsafeAtomicAdd(dst + x, v);
+            float *dst = imgf + gp->img_off[0] + (long long)ld_l * img_ld;
+            const float4 *e0 = src + fps0;
+            const unsigned xend = (unsigned)nx * 4u;
+            for (unsigned xo = (unsigned)tid * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
+                const float4 *e = e0;
+                float4 g = *e;
+                float v;
+                if constexpr (EXACT)
+                    v = rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                else
+                    v = fmaf(g.y, g.w, g.x * g.z);
+                for (int w = 1; w < W; ++w) {
+                    e += rowlen;
+                    g = *e;
+                    if constexpr (EXACT)
+                        v += rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                    else
+                        v = fmaf(g.y, g.w, fmaf(g.x, g.z, v));
                 }
+                unsafeAtomicAdd(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
             }
         };
         using c0 = std::integral_constant<int, 0>;

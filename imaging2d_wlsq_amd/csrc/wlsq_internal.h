@@ -11,6 +11,9 @@
 namespace wlsq {
 
 constexpr int kMaxShotsPerGroup = 4;
+// Largest kappa sample count: below 2^22 the rounding step of the lookup is
+// exact (u - 0.5 and 0.5 - u are representable for 0 <= u < 2^22).
+constexpr int kMaxKappaSamples = 1 << 22;
 constexpr int kMaxImagePlanes = 16;
 
 // One shot group: up to 4 shots that share aperture, depth count and
