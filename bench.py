@@ -332,6 +332,9 @@ def bench_c5(args, log):
 def bench_serving(args, log):
     """Continuous admission of jobs of mixed size against 4 resident models.
 
+    --serve-mode rounds (default): continuous batching, the running jobs of a
+    model advanced together by round launches that later jobs join;
+    streams: one launch per job on one of --serve-slots streams.
     A saturation pass (the first jobs all arrive at once) measures the loop's
     service rate; the timed run then draws Poisson arrivals at
     --serve-load times that rate."""
@@ -341,25 +344,46 @@ def bench_serving(args, log):
 
     t0 = time.perf_counter()
     models = serve.serving_models()
-    launcher = serve.GpuLauncher(models, args.serve_slots)
+    budget = 32 << 30
+    rounds = args.serve_mode == "rounds"
+    if rounds:
+        launcher = serve.GpuRoundEngine(models, hbm_budget=budget)
+        mk = lambda: serve.RoundServer(launcher, round_points=args.serve_round_points,  # noqa: E731
+                                       max_active=args.serve_max_active, lanes=args.serve_lanes,
+                                       min_steps=args.serve_min_steps)
+    else:
+        launcher = serve.GpuLauncher(models, args.serve_slots)
+        mk = lambda: serve.Server(launcher, hbm_budget=budget)  # noqa: E731
     setup = time.perf_counter() - t0
     nzm, nxm = models[0].model.nz, models[0].model.nx
-    budget = 32 << 30
     calib = serve.make_jobs(min(48, args.serve_jobs), 1.0, len(models), nzm, nxm, seed=4)
     for j in calib:
         j.arrival = 0.0
-    cst = serve.Server(launcher, hbm_budget=budget).run(calib)
+    cst = mk().run(calib)
     cap_pts = cst.points / cst.wall
+    log(f"serving saturation pass: {len(calib)} jobs at once, {cap_pts:.3e} points/s, "
+        f"p50 {cst.summary()['p50_latency_s']:.3f} s")
     probe = serve.make_jobs(2000, 1.0, len(models), nzm, nxm, seed=4)
     mean_pts = float(np.mean([j.points for j in probe]))
     rate = args.serve_load * cap_pts / mean_pts
     jobs = serve.make_jobs(args.serve_jobs, rate, len(models), nzm, nxm, seed=4)
-    st = serve.Server(launcher, hbm_budget=budget).run(
-        jobs, log=log if args.serve_jobs >= 500 else None, log_every=100)
+    srv = mk()
+    st = srv.run(jobs, log=log if args.serve_jobs >= 500 else None, log_every=100)
     launcher.close()
     s = st.summary()
+    if rounds:
+        n_in = [len(m) for _, m, _, _ in srv.round_log]
+        s.update({"mode": "continuous batching: rounds of one fused launch per model and kernel "
+                          "shape, jobs joining between rounds",
+                  "rounds": srv.rounds_launched, "mean_jobs_per_round": float(np.mean(n_in)),
+                  "rounds_with_jobs_at_different_depths":
+                      sum(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log),
+                  "round_points": args.serve_round_points, "max_active": args.serve_max_active,
+                  "lanes": args.serve_lanes, "min_steps": args.serve_min_steps})
+    else:
+        s.update({"mode": "one launch per job on its own stream", "slots": args.serve_slots})
     s.update({"arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,
-              "saturated_points_per_s": cap_pts, "slots": args.serve_slots,
+              "saturated_points_per_s": cap_pts,
               "hbm_budget_bytes": budget, "models": len(models), "model_shape": [nzm, nxm],
               "setup_s": setup,
               "note": f"first {args.serve_jobs} jobs of the seeded 2000-job stream; arrival rate = "
@@ -393,7 +417,16 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the serving run (0: skip; the configuration is 2000)")
-    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-slots", type=int, default=8, help="streams of --serve-mode streams")
+    ap.add_argument("--serve-mode", choices=["rounds", "streams"], default="rounds")
+    ap.add_argument("--serve-round-points", type=float, default=3e9,
+                    help="points per round of --serve-mode rounds")
+    ap.add_argument("--serve-max-active", type=int, default=24,
+                    help="running jobs at most (--serve-mode rounds)")
+    ap.add_argument("--serve-lanes", type=int, default=4,
+                    help="round sequences per cohort (--serve-mode rounds)")
+    ap.add_argument("--serve-min-steps", type=int, default=64,
+                    help="depth steps per round at least (--serve-mode rounds)")
     ap.add_argument("--c5-shots", type=int, default=125,
                     help="shots of the 2000 x 8192 survey measured in one launch (0: skip; 125 is "
                          "one GPU's share of the 1000-shot survey over 8 GPUs)")

@@ -27,7 +27,7 @@ EXPORTS = [
     "wlsq_plan_create", "wlsq_plan_info", "wlsq_plan_launch", "wlsq_plan_destroy",
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
     "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device", "wlsq_ktable_device_noalloc",
-    "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws", "wlsq_probe_fma",
+    "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws", "wlsq_probe_fma", "wlsq_batch_round_ws",
 ]
 
 
@@ -49,7 +49,7 @@ class Shot(Structure):
     _fields_ = [("f_off", c_longlong), ("b_off", c_longlong), ("img_off", c_longlong),
                 ("slow_off", c_longlong), ("ld", c_int), ("img_ld", c_int), ("slow_ld", c_int),
                 ("nx", c_int), ("nz", c_int), ("nf", c_int), ("omega_off", c_int),
-                ("reserved", c_int)]
+                ("first_step", c_int)]
 
 
 _c64p = np.ctypeslib.ndpointer(dtype=np.complex64, flags=("C", "A"))
@@ -95,6 +95,9 @@ def lib():
     L.wlsq_batch_device_ws.restype = ci
     L.wlsq_batch_device_ws.argtypes = [POINTER(Shot), ci, ci, vp, vp, cf, cf, ci, vp, vp, vp, vp,
                                        vp, ci, POINTER(Tuning), vp, c_longlong, vp]
+    L.wlsq_batch_round_ws.restype = ci
+    L.wlsq_batch_round_ws.argtypes = [POINTER(Shot), ci, ci, ci, vp, vp, cf, cf, ci, vp, vp, vp,
+                                      vp, vp, POINTER(Tuning), vp, c_longlong, vp]
     L.wlsq_plan_create.restype = ci
     L.wlsq_plan_create.argtypes = [POINTER(vp), POINTER(Shot), ci, ci, POINTER(Tuning)]
     L.wlsq_plan_info.restype = ci

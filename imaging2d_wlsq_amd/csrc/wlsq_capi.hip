@@ -257,7 +257,7 @@ struct PlanImpl {
     int strip_gathers = 1;
     int double_rows = 0;
     int depth_steps = 0;         // depth steps per launch (0: all)
-    int max_nz = 0;
+    int max_nz = 0;              // most depth steps a shot runs in one launch
     long long f_extent = 0, b_extent = 0;  // elements of the F and B pools the shots use
     DevBuf fields;               // copies of the pools for split launches without write-back
     long long points = 0;
@@ -308,14 +308,26 @@ bool same_shape(const wlsq_shot &a, const wlsq_shot &b)
 {
     return a.nx == b.nx && a.nz == b.nz && a.nf == b.nf && a.omega_off == b.omega_off &&
            a.slow_off == b.slow_off && a.slow_ld == b.slow_ld && a.ld == b.ld &&
-           a.img_ld == b.img_ld;
+           a.img_ld == b.img_ld && a.first_step == b.first_step;
+}
+
+// Depth steps shot `s` runs in a launch of a plan: every step (round_steps
+// 0), or one round of a continuous batch, [first_step, first_step + round_steps)
+// cut at nz.
+long long shot_steps(const wlsq_shot &s, int round_steps)
+{
+    if (round_steps <= 0)
+        return s.nz;
+    return std::max(0ll, std::min<long long>(s.nz, (long long)s.first_step + round_steps) - s.first_step);
 }
 
 // The work list and launch shape of a batch; with `upload` the group and
 // task descriptors are copied to plan-owned device memory (the allocation-free
-// entry places them itself).
+// entry places them itself). round_steps > 0: one round of a continuous batch,
+// each shot running round_steps steps from its first_step (shots with nothing
+// left have no tasks); round_steps 0: every first_step must be 0.
 int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_tuning *tun,
-               std::shared_ptr<PlanImpl> &out, bool upload = true)
+               std::shared_ptr<PlanImpl> &out, bool upload = true, int round_steps = 0)
 {
     if (nshots < 0 || (nshots > 0 && !shots))
         return fail(WLSQ_EINVAL, "shots array is NULL");
@@ -333,15 +345,18 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     for (int i = 0; i < nshots; ++i) {
         const wlsq_shot &s = shots[i];
         if (s.nx < 0 || s.nz < 0 || s.nf < 0 || s.omega_off < 0 || s.f_off < 0 || s.b_off < 0 ||
-            s.img_off < 0 || s.slow_off < 0)
+            s.img_off < 0 || s.slow_off < 0 || s.first_step < 0)
             return fail(WLSQ_EINVAL, "shot %d: negative size or offset", i);
-        if (s.nx > 0 && s.nf > 0 && s.nz > 0) {
+        if (round_steps <= 0 && s.first_step != 0)
+            return fail(WLSQ_EINVAL, "shot %d: first_step %d (only rounds of a continuous batch "
+                                     "start below the surface)", i, s.first_step);
+        if (s.nx > 0 && s.nf > 0 && shot_steps(s, round_steps) > 0) {
             if (s.ld < s.nx || s.img_ld < s.nx || s.slow_ld < s.nx)
                 return fail(WLSQ_EINVAL, "shot %d: row stride below aperture (ld=%d img_ld=%d slow_ld=%d nx=%d)",
                             i, s.ld, s.img_ld, s.slow_ld, s.nx);
             max_nx = std::max(max_nx, s.nx);
         }
-        P->points += (long long)s.nx * s.nz * s.nf;
+        P->points += (long long)s.nx * shot_steps(s, round_steps) * s.nf;
     }
 
     // longest run of consecutive same-shaped shots (they can share a group)
@@ -419,7 +434,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     std::vector<std::pair<int, int>> runs;  // (first shot, count)
     for (int i = 0; i < nshots;) {
         const wlsq_shot &s = shots[i];
-        if (s.nx == 0 || s.nf == 0 || s.nz == 0) {
+        if (s.nx == 0 || s.nf == 0 || shot_steps(s, round_steps) == 0) {
             ++i;
             continue;
         }
@@ -488,7 +503,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         g.nf = s.nf;
         g.omega_off = s.omega_off;
         g.img_shared = run_shared(rn) ? 1 : 0;
-        g.pad_ = 0;
+        g.z0 = s.first_step;
         const int img_rows = img_lds == 1 ? (g.img_shared ? 1 : S) : 0;
         const int strips = (s.nx + R - 1) / R;
         int W = block * C / strips;
@@ -513,7 +528,7 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
             pd.item.j0 = j0;
             pd.item.nw = std::min(W, s.nf - j0);
             pd.item.cost_rank = 0;
-            pd.cost = (long long)s.nz * s.nx * pd.item.nw * rn.second;
+            pd.cost = shot_steps(s, round_steps) * s.nx * pd.item.nw * rn.second;
             pend.push_back(pd);
         }
     }
@@ -563,9 +578,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     P->depth_steps = t.depth_steps;
     for (int i = 0; i < nshots; ++i) {
         const wlsq_shot &sh = shots[i];
-        if (sh.nx == 0 || sh.nf == 0 || sh.nz == 0)
+        if (sh.nx == 0 || sh.nf == 0 || shot_steps(sh, round_steps) == 0)
             continue;
-        P->max_nz = std::max(P->max_nz, sh.nz);
+        P->max_nz = std::max(P->max_nz, (int)shot_steps(sh, round_steps));
         P->f_extent = std::max(P->f_extent, sh.f_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
         P->b_extent = std::max(P->b_extent, sh.b_off + (long long)(sh.nf - 1) * sh.ld + sh.nx);
         P->img_extent = std::max(P->img_extent, sh.img_off + (long long)(sh.nz - 1) * sh.img_ld + sh.nx);
@@ -1084,7 +1099,9 @@ long long wlsq_batch_workspace_bytes(const wlsq_shot *shots, int nshots, int M, 
     if (current_device(&d) != WLSQ_OK)
         return -1;
     std::shared_ptr<PlanImpl> P;
-    if (build_plan(d, shots, nshots, M, &t, P, /*upload=*/false) != WLSQ_OK)
+    // the task list does not depend on the steps of a round: size it as a
+    // round of every remaining step (the full launch when all first_step are 0)
+    if (build_plan(d, shots, nshots, M, &t, P, /*upload=*/false, kMaxRoundSteps) != WLSQ_OK)
         return -1;
     return (long long)ws_layout(*P, nk).need;
 }
@@ -1116,6 +1133,38 @@ int wlsq_batch_device_ws(const wlsq_shot *shots, int nshots, int M, const float 
     return launch_in_workspace(*P, static_cast<unsigned char *>(workspace), w, slowness, omega,
                                kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
                                write_back, static_cast<hipStream_t>(stream));
+}
+
+int wlsq_batch_round_ws(const wlsq_shot *shots, int nshots, int steps, int M,
+                        const float *slowness, const float *omega, float kappa0, float dkappa,
+                        int nk, const wlsq_c32 *table_forw, const wlsq_c32 *table_back,
+                        wlsq_c32 *forw, wlsq_c32 *back, float *image, const wlsq_tuning *tuning,
+                        void *workspace, long long workspace_bytes, void *stream)
+{
+    if (steps < 1 || steps > kMaxRoundSteps)
+        return fail(WLSQ_EINVAL, "steps must be 1..%d (got %d)", kMaxRoundSteps, steps);
+    wlsq_tuning t{};
+    WLSQ_TRY(workspace_tuning(tuning, &t));
+    Device *d = nullptr;
+    WLSQ_TRY(current_device(&d));
+    std::shared_ptr<PlanImpl> P;
+    WLSQ_TRY(build_plan(d, shots, nshots, M, &t, P, /*upload=*/false, steps));
+    if (P->items.empty())
+        return WLSQ_OK;
+    if (!slowness || !omega || !table_forw || (!table_back && !P->shape.conj) || !forw || !back ||
+        !image || !workspace)
+        return fail(WLSQ_EINVAL, "NULL device pointer");
+    WLSQ_TRY(check_kappa(nk, dkappa, P->L));
+    if (reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+        return fail(WLSQ_EINVAL, "workspace must be 256-byte aligned");
+    const WsLayout w = ws_layout(*P, nk);
+    if (workspace_bytes < 0 || (size_t)workspace_bytes < w.need)
+        return fail(WLSQ_ENOMEM, "workspace of %lld bytes, the round needs %zu", workspace_bytes,
+                    w.need);
+    // the fields carry each shot's state from one round to the next
+    return launch_in_workspace(*P, static_cast<unsigned char *>(workspace), w, slowness, omega,
+                               kappa0, dkappa, nk, table_forw, table_back, forw, back, image,
+                               /*write_back=*/1, static_cast<hipStream_t>(stream));
 }
 
 int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

@@ -99,9 +99,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     if (W * nstrips > C * NT)
         return;  // the planner never builds such a task
     // depth steps of this launch: [lb, le) of the task's nz (a plan may split
-    // the depth range over several launches; the fields carry the state)
-    const int lb = a.l0;
-    const int le = min(nz, a.l1);
+    // the depth range over several launches, and a round of a continuous
+    // batch starts each group at its own step z0; the fields carry the state)
+    const int lb = gp->z0 + a.l0;
+    const int le = min(nz, gp->z0 + a.l1);
     if (lb >= le)
         return;  // this task's depth range ended in an earlier launch
     // With double_rows the task has two copies of its rows and of its image

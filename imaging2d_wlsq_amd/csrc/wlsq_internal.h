@@ -15,6 +15,8 @@ constexpr int kMaxShotsPerGroup = 4;
 // exact (u - 0.5 and 0.5 - u are representable for 0 <= u < 2^22).
 constexpr int kMaxKappaSamples = 1 << 22;
 constexpr int kMaxImagePlanes = 16;
+// Most depth steps of one round of a continuous batch (wlsq_batch_round_ws).
+constexpr int kMaxRoundSteps = 1 << 24;
 
 // One shot group: up to 4 shots that share aperture, depth count and
 // frequency list, so one coefficient load serves all of them.
@@ -32,11 +34,11 @@ struct Group {
     int nf;                               // frequencies
     int omega_off;                        // first frequency in the omega pool
     int img_shared;                       // all shots add into the same image rows
-    int pad_;
+    int z0;                               // first depth step: a launch runs [z0 + l0, z0 + l1)
 };
 
 // One workgroup task: `nw` consecutive frequencies (from `j0`) of one group,
-// all depth steps.
+// the depth steps of the launch.
 struct Item {
     int group;
     int j0;
@@ -68,7 +70,8 @@ struct FusedArgs {
     int nplanes;
     int strip_gathers;  // 1: one or two gathers per strip where the kappa rows allow it
     int double_rows;    // 1: two copies of the rows in LDS, one barrier per step
-    int l0, l1;         // depth steps [l0, min(nz, l1)) of each task in this launch
+    int l0, l1;         // depth steps [z0 + l0, min(nz, z0 + l1)) of each task in this
+                        // launch (z0 of the task's group)
 };
 
 // Launch shape of the fused kernel; chosen by the planner.

@@ -180,6 +180,7 @@ class ShotSpec:
     nz: int
     nf: int
     omega_off: int
+    first_step: int = 0  # rounds of a continuous batch: the next depth step
 
 
 def check_shot_extents(shots: Sequence[ShotSpec], M: int, nk: int, slowness, omega, table_forw,
@@ -206,7 +207,7 @@ def _shots_array(shots: Sequence[ShotSpec]):
     arr = (Shot * max(1, len(shots)))()
     for i, s in enumerate(shots):
         arr[i] = Shot(s.f_off, s.b_off, s.img_off, s.slow_off, s.ld, s.img_ld, s.slow_ld, s.nx,
-                      s.nz, s.nf, s.omega_off, 0)
+                      s.nz, s.nf, s.omega_off, s.first_step)
     return arr
 
 
@@ -242,6 +243,33 @@ def batch_launch_ws(shots: Sequence[ShotSpec], M: int, slowness, omega, kappa0, 
                                      back.data_ptr(), image.data_ptr(), int(bool(write_back)),
                                      byref(tun), workspace.data_ptr(), int(workspace.numel()),
                                      _stream(stream)))
+
+
+def batch_round_ws(shots: Sequence[ShotSpec], steps: int, M: int, slowness, omega, kappa0, dkappa,
+                   nk, table_forw, table_back, forw, back, image, workspace: torch.Tensor,
+                   tuning: Optional[dict] = None, stream=None, check_extents: bool = True) -> None:
+    """Queue one round of a continuous batch: every shot runs `steps` depth
+    steps from its ``first_step`` (cut at its nz) on the wavefields its rows
+    hold, writes them back and adds its image rows. Descriptors and table
+    copies go into `workspace` (at least :func:`batch_workspace_bytes` of the
+    same shots); no device allocation, no wait. ``check_extents=False`` skips
+    the per-shot extent checks for callers that placed every shot inside the
+    tensors themselves (the serving loop's pools)."""
+    c = torch.complex64
+    for t, dt, name in ((slowness, torch.float32, "slowness"), (omega, torch.float32, "omega"),
+                        (table_forw, c, "table_forw"), (table_back, c, "table_back"),
+                        (forw, c, "forw"), (back, c, "back"), (image, torch.float32, "image"),
+                        (workspace, torch.uint8, "workspace")):
+        _ptr(t, dt, name)
+    if check_extents:
+        check_shot_extents(shots, M, nk, slowness, omega, table_forw, table_back, forw, back, image)
+    tun = _tuning(tuning)
+    check(lib().wlsq_batch_round_ws(_shots_array(shots), len(shots), int(steps), M,
+                                    slowness.data_ptr(), omega.data_ptr(), float(kappa0),
+                                    float(dkappa), int(nk), table_forw.data_ptr(),
+                                    table_back.data_ptr(), forw.data_ptr(), back.data_ptr(),
+                                    image.data_ptr(), byref(tun), workspace.data_ptr(),
+                                    int(workspace.numel()), _stream(stream)))
 
 
 class Plan:

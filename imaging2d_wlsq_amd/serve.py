@@ -1,8 +1,20 @@
-"""Serving loop: continuous admission of migration jobs of mixed size.
+"""Serving loops: continuous admission of migration jobs of mixed size.
 
 Jobs arrive by a Poisson process. Each job names one of the velocity models
 kept resident on the GPU (slowness + compact tables), an aperture window, a
-depth count, a frequency list and 1-32 shots. The loop
+depth count, a frequency list and 1-32 shots.
+
+:class:`RoundServer` (continuous batching, the default): the running jobs of a
+model advance together in rounds. A round is ONE fused launch that moves
+every running job's shots by a number of depth steps from where each one is
+(``wlsq_batch_round_ws``); the wavefields stay in HBM between rounds, so a
+job admitted while others run joins the very next round of its cohort (its
+model's operator tables and its kernel shape) and shares its launches, and a
+finished job leaves the batch at its round's end.
+Jobs are admitted first come first served as soon as their wavefields and
+image fit the device pools (the HBM budget).
+
+:class:`Server` with :class:`GpuLauncher` (one launch per job): the loop
 
 * admits an arrived job as soon as a slot (a HIP stream) is free and its HBM
   footprint (wavefields + stacked image + work list) fits the remaining
@@ -18,7 +30,7 @@ depth count, a frequency list and 1-32 shots. The loop
 
 Jobs admitted while others run overlap on the device (their launches share
 the CUs through the hardware queues), so the GPU stays busy across job
-boundaries: continuous batching at the granularity of workgroup tasks.
+boundaries.
 """
 from __future__ import annotations
 
@@ -256,3 +268,372 @@ class Server:
                 idle()
         st.wall = clock() - t0
         return st
+
+
+# --------------------------------------------------------------- continuous batching
+def job_shape(job: Job) -> tuple:
+    """(x per thread, shots per group) the library's planner picks for this
+    job launched alone (2 x per thread for groups of shots up to 1024 wide,
+    else 4; two shots per group up to 2048, else one; the CPU path's rounding
+    is compiled for 4 x and one shot). Jobs of one model and shape share
+    rounds, so a job runs the same kernel shape alone and in the mix."""
+    if job.rounding:
+        return (4, 1)
+    if job.nshots >= 2 and job.nx <= 1024:
+        return (2, 4 if job.nshots >= 8 else 2)
+    return (4, 2 if job.nshots >= 2 and job.nx <= 2048 else 1)
+
+
+class RangePool:
+    """First-fit allocator of element ranges [0, n) (offsets aligned to
+    `align`); freed ranges merge with their neighbours."""
+
+    def __init__(self, n: int, align: int = 64):
+        self.n = int(n)
+        self.align = int(align)
+        self.free: List[List[int]] = [[0, self.n]] if self.n > 0 else []
+        self.used = 0
+
+    def _round(self, k: int) -> int:
+        return -(-int(k) // self.align) * self.align
+
+    def alloc(self, k: int) -> Optional[int]:
+        k = self._round(max(1, k))
+        for i, (off, size) in enumerate(self.free):
+            if size >= k:
+                if size == k:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = [off + k, size - k]
+                self.used += k
+                return off
+        return None
+
+    def release(self, off: int, k: int) -> None:
+        k = self._round(max(1, k))
+        self.used -= k
+        i = 0
+        while i < len(self.free) and self.free[i][0] < off:
+            i += 1
+        self.free.insert(i, [off, k])
+        # merge with the right, then the left neighbour
+        if i + 1 < len(self.free) and self.free[i][0] + self.free[i][1] == self.free[i + 1][0]:
+            self.free[i][1] += self.free[i + 1][1]
+            self.free.pop(i + 1)
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == self.free[i][0]:
+            self.free[i - 1][1] += self.free[i][1]
+            self.free.pop(i)
+
+
+@dataclass
+class _RoundJob:
+    job: Job
+    key: tuple
+    state: object          # the engine's per-job data (pool ranges, shot descriptors)
+    t_admit: float
+    z: int = 0             # next depth step
+    done_round: int = -1   # round after which the job is complete
+
+
+@dataclass
+class _Round:
+    rid: int
+    handle: object
+    jobs: List[_RoundJob]
+    steps: int
+    points: int
+
+
+class RoundServer:
+    """Continuous batching over an engine (GPU, or a fake one in CPU tests).
+
+    Per cohort (model and kernel shape) at most `depth` rounds are queued on
+    the device; each round advances every running job of the cohort by
+    `steps` depth steps, chosen so a round holds about `round_points` points
+    (between `min_steps` and `max_steps`, a multiple of `min_steps`). Arrived
+    jobs are admitted first come first served while fewer than `max_active`
+    run and the engine can place their wavefields and image."""
+
+    def __init__(self, engine, round_points: float = 3e9, min_steps: int = 16,
+                 max_steps: int = 512, depth: int = 2, max_active: int = 24, lanes: int = 1):
+        self.engine = engine
+        # jobs of one cohort are spread over `lanes` round sequences (the lane
+        # with the least work left takes a new job); the lanes' rounds are
+        # independent, so one lane's round can fill the tail of another's
+        self.lanes = max(1, int(lanes))
+        self.round_points = float(round_points)
+        self.min_steps = int(min_steps)
+        self.max_steps = int(max_steps)
+        self.depth = int(depth)
+        self.max_active = int(max_active)
+        self.rounds_launched = 0
+        # every round: (cohort, [(job id, first step)], steps, points)
+        self.round_log: List[tuple] = []
+
+    def _steps(self, jobs: List[_RoundJob]) -> int:
+        per_step = sum(r.job.nx * r.job.nf * r.job.nshots for r in jobs)
+        d = int(self.round_points // max(1, per_step))
+        d = max(self.min_steps, min(self.max_steps, d))
+        return max(self.min_steps, d // self.min_steps * self.min_steps)
+
+    def run(self, jobs: Sequence[Job], keep_images: bool = False, time_scale: float = 1.0,
+            log=None, log_every: int = 1, clock=time.perf_counter,
+            idle=lambda: time.sleep(2e-4)) -> ServeStats:
+        """Serve `jobs` (arrival times in seconds, divided by `time_scale`);
+        `log` gets a line every `log_every` completed jobs."""
+        eng = self.engine
+        st = ServeStats()
+        for j in jobs:
+            if not eng.fits(j):
+                raise ValueError(f"job {j.job_id} does not fit the device pools")
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        waiting: deque = deque()
+        cohorts: Dict[tuple, List[_RoundJob]] = {}
+        inflight: Dict[tuple, deque] = {}
+        n_active = 0
+        eng.sync()
+        t0 = clock()
+        while pending or waiting or n_active:
+            now = clock() - t0
+            progressed = False
+            while pending and pending[0].arrival / time_scale <= now:
+                waiting.append(pending.popleft())
+                progressed = True
+            # completed rounds: their finished jobs leave
+            for key, q in inflight.items():
+                while q and eng.done(q[0].handle):
+                    rd = q.popleft()
+                    progressed = True
+                    for r in rd.jobs:
+                        if r.done_round != rd.rid:
+                            continue
+                        t_done = clock() - t0
+                        st.latencies.append(t_done - r.job.arrival / time_scale)
+                        st.order.append(r.job.job_id)
+                        st.points += r.job.points
+                        if keep_images:
+                            st.images[r.job.job_id] = eng.image(r.state)
+                        eng.release(r.state)
+                        cohorts[key].remove(r)
+                        n_active -= 1
+                        if log and len(st.latencies) % max(1, log_every) == 0:
+                            log(f"job {r.job.job_id} done ({len(st.latencies)} of {len(jobs)}): "
+                                f"{r.job.points:.3g} pts, latency {st.latencies[-1]:.3f} s")
+            # admission: first come first served, while the pools place the job
+            while waiting and n_active < self.max_active:
+                job = waiting[0]
+                state = eng.admit(job)
+                if state is None:
+                    break  # wait for memory to free up
+                waiting.popleft()
+                t_adm = clock() - t0
+                st.queue_waits.append(t_adm - job.arrival / time_scale)
+                base = eng.cohort(job)
+                left = lambda k: sum(r.job.nx * r.job.nf * r.job.nshots * (r.job.nz - r.z)  # noqa: E731
+                                     for r in cohorts.get(k, []))
+                key = min((base + (ln,) for ln in range(self.lanes)), key=left)
+                cohorts.setdefault(key, []).append(_RoundJob(job, key, state, t_adm))
+                inflight.setdefault(key, deque())
+                n_active += 1
+                st.max_active = max(st.max_active, n_active)
+                st.peak_bytes = max(st.peak_bytes, eng.bytes_in_use())
+                progressed = True
+            # rounds: every cohort keeps up to `depth` rounds queued
+            for key, members in cohorts.items():
+                q = inflight[key]
+                while len(q) < self.depth:
+                    run = [r for r in members if r.z < r.job.nz]
+                    if not run:
+                        break
+                    steps = self._steps(run)
+                    rid = self.rounds_launched
+                    self.rounds_launched += 1
+                    pts = sum(r.job.nx * r.job.nf * r.job.nshots * (min(r.job.nz, r.z + steps) - r.z)
+                              for r in run)
+                    h = eng.launch_round(key, [(r.state, r.z) for r in run], steps)
+                    self.round_log.append((key, [(r.job.job_id, r.z) for r in run], steps, pts))
+                    for r in run:
+                        r.z = min(r.job.nz, r.z + steps)
+                        if r.z >= r.job.nz:
+                            r.done_round = rid
+                    q.append(_Round(rid, h, run, steps, pts))
+                    progressed = True
+            if not progressed:
+                idle()
+        st.wall = clock() - t0
+        return st
+
+
+class _PoolJob:
+    """A job's place in the engine's pools and its shot descriptors."""
+
+    def __init__(self, job, f_off, f_len, i_off, i_len, o_off, specs, nbytes):
+        self.job = job
+        self.f_off, self.f_len = f_off, f_len
+        self.i_off, self.i_len = i_off, i_len
+        self.o_off = o_off
+        self.specs = specs
+        self.nbytes = nbytes
+
+
+class GpuRoundEngine:
+    """Device side of :class:`RoundServer`: the resident models, three device
+    pools (wavefields, images, frequency lists) carved out of the HBM budget,
+    one HIP stream and two launch workspaces per cohort (table family, kernel
+    shape, rounding). Models whose operator tables and kappa grids are equal
+    bit for bit form one table family: their slowness models sit side by side
+    in one device array, so jobs on any of them share rounds. A job's
+    wavefields are generated in its pool range on its cohort's stream; a
+    round is one ``wlsq_batch_round_ws`` launch."""
+
+    def __init__(self, models: Sequence[synth.Problem], hbm_budget: int = 32 << 30, dev="cuda",
+                 image_share: float = 0.1, omega_slots: int = 1 << 20):
+        self.surveys = [batch.Survey(p, dev) for p in models]
+        self.dev = torch.device(dev)
+        self.M = models[0].tables.M
+        # table families: the first model of each family holds its tables
+        self.family: List[int] = []
+        for i, p in enumerate(models):
+            t = p.tables
+            fam = i
+            for k in range(i):
+                u = models[k].tables
+                if (self.family[k] == k and (t.M, t.nk, t.kappa0, t.dkappa) == (u.M, u.nk, u.kappa0, u.dkappa)
+                        and np.array_equal(t.forw.view(np.uint32), u.forw.view(np.uint32))
+                        and np.array_equal(t.back.view(np.uint32), u.back.view(np.uint32))):
+                    fam = k
+                    break
+            self.family.append(fam)
+        # the slowness models of a family in one array: model i at slow_base[i]
+        self.slow_base: List[int] = [0] * len(models)
+        self.family_slowness: Dict[int, torch.Tensor] = {}
+        for fam in sorted(set(self.family)):
+            members = [i for i, f in enumerate(self.family) if f == fam]
+            o = 0
+            for i in members:
+                self.slow_base[i] = o
+                o += models[i].model.slowness.size
+            self.family_slowness[fam] = torch.cat(
+                [self.surveys[i].slowness.reshape(-1) for i in members]).contiguous()
+        img_bytes = int(hbm_budget * image_share)
+        self.fpool = RangePool((int(hbm_budget) - img_bytes) // 8)
+        self.ipool = RangePool(img_bytes // 4)
+        self.opool = RangePool(omega_slots, align=16)
+        self.fields = torch.empty(self.fpool.n, dtype=torch.complex64, device=self.dev)
+        self.images = torch.zeros(self.ipool.n, dtype=torch.float32, device=self.dev)
+        self.omegas = torch.zeros(self.opool.n, dtype=torch.float32, device=self.dev)
+        self.streams: Dict[tuple, torch.cuda.Stream] = {}
+        self.ws: Dict[tuple, List[Optional[torch.Tensor]]] = {}
+        self.flip: Dict[tuple, int] = {}
+
+    def cohort(self, job: Job) -> tuple:
+        return (self.family[job.model],) + job_shape(job) + (job.rounding,)
+
+    def fits(self, job: Job) -> bool:
+        return (2 * job.nshots * job.nf * job.nx <= self.fpool.n and job.nz * job.nx <= self.ipool.n
+                and job.nf <= self.opool.n)
+
+    def bytes_in_use(self) -> int:
+        return 8 * self.fpool.used + 4 * self.ipool.used
+
+    def _stream(self, key):
+        if key not in self.streams:
+            self.streams[key] = torch.cuda.Stream(device=self.dev)
+            self.ws[key] = [None, None]
+            self.flip[key] = 0
+        return self.streams[key]
+
+    def admit(self, job: Job):
+        n = job.nshots * job.nf * job.nx
+        f_off = self.fpool.alloc(2 * n)
+        if f_off is None:
+            return None
+        i_off = self.ipool.alloc(job.nz * job.nx)
+        if i_off is None:
+            self.fpool.release(f_off, 2 * n)
+            return None
+        o_off = self.opool.alloc(job.nf)
+        if o_off is None:
+            self.fpool.release(f_off, 2 * n)
+            self.ipool.release(i_off, job.nz * job.nx)
+            return None
+        p = self.surveys[job.model].problem
+        shots = job.shots()
+        key = self.cohort(job)
+        stream = self._stream(key)
+        with torch.cuda.stream(stream):
+            # F of the job's shots in the first half of its range, B in the second
+            _, _, offs = synth.device_fields(p, shots, device=self.dev,
+                                             out_f=self.fields[f_off:f_off + n],
+                                             out_b=self.fields[f_off + n:f_off + 2 * n])
+            self.images[i_off:i_off + job.nz * job.nx].zero_()
+            self.omegas[o_off:o_off + job.nf].copy_(torch.from_numpy(shots[0].omega))
+        specs = [device.ShotSpec(f_off=f_off + o, b_off=f_off + n + o, img_off=i_off,
+                                 slow_off=self.slow_base[job.model] + job.x0, ld=job.nx,
+                                 img_ld=job.nx, slow_ld=p.model.nx,
+                                 nx=job.nx, nz=job.nz, nf=job.nf, omega_off=o_off) for o in offs]
+        return _PoolJob(job, f_off, 2 * n, i_off, job.nz * job.nx, o_off, specs,
+                        8 * 2 * n + 4 * job.nz * job.nx)
+
+    def _tuning(self, key) -> dict:
+        fam, R, S, rounding = key[:4]
+        sv = self.surveys[fam]
+        t = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym), x_per_thread=R,
+                 shots_per_group=S)
+        if rounding:
+            t["rounding"] = 1
+        return t
+
+    def _ws_bytes(self, specs, nk) -> int:
+        """Upper bound of wlsq_batch_workspace_bytes: a group per shot (144
+        bytes), a task per (shot, frequency) (16 bytes), two row-padded
+        table copies, each part 256-byte aligned."""
+        al = lambda b: (b + 255) // 256 * 256  # noqa: E731
+        L = 2 * self.M + 1
+        return (al(144 * len(specs)) + al(16 * sum(s.nf for s in specs)) +
+                2 * al(8 * (L + 1) * nk) + 256)
+
+    def launch_round(self, key, members, steps):
+        fam = key[0]
+        sv = self.surveys[fam]
+        t = sv.problem.tables
+        specs = []
+        for state, z in members:
+            for sp in state.specs:
+                sp.first_step = z
+                specs.append(sp)
+        stream = self._stream(key)
+        k = self.flip[key]
+        self.flip[key] = k ^ 1
+        need = self._ws_bytes(specs, t.nk)
+        ws = self.ws[key][k]
+        if ws is None or ws.numel() < need:
+            with torch.cuda.stream(stream):
+                ws = torch.empty(max(need, 4 << 20), dtype=torch.uint8, device=self.dev)
+            self.ws[key][k] = ws
+        device.batch_round_ws(specs, steps, t.M, self.family_slowness[fam], self.omegas, t.kappa0,
+                              t.dkappa, t.nk,
+                              sv.tf, sv.tb, self.fields, self.fields, self.images, ws,
+                              tuning=self._tuning(key), stream=stream, check_extents=False)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return ev
+
+    def done(self, handle) -> bool:
+        return handle.query()
+
+    def image(self, state) -> np.ndarray:
+        j = state.job
+        return self.images[state.i_off:state.i_off + j.nz * j.nx].view(j.nz, j.nx).cpu().numpy()
+
+    def release(self, state) -> None:
+        self.fpool.release(state.f_off, state.f_len)
+        self.ipool.release(state.i_off, state.i_len)
+        self.opool.release(state.o_off, state.job.nf)
+
+    def sync(self):
+        torch.cuda.synchronize()
+
+    def close(self):
+        torch.cuda.synchronize()
+        self.ws = {k: [None, None] for k in self.ws}

@@ -247,7 +247,9 @@ typedef struct {
     int nz;                 /* depth steps */
     int nf;                 /* frequencies */
     int omega_off;          /* offset of the frequency list in omega */
-    int reserved;
+    int first_step;         /* wlsq_batch_round_ws: the shot's next depth step (its
+                               fields hold the wavefields after step first_step - 1);
+                               every other entry: must be 0 */
 } wlsq_shot;
 
 /* One call: plan, launch, wait, free (synchronous). Adds into `image`. */
@@ -273,6 +275,22 @@ int wlsq_batch_device_ws(const wlsq_shot *shots, int nshots, int M, const float 
                          const wlsq_c32 *table_forw, const wlsq_c32 *table_back, wlsq_c32 *forw,
                          wlsq_c32 *back, float *image, int write_back, const wlsq_tuning *tuning,
                          void *workspace, long long workspace_bytes, void *stream);
+
+/* One round of continuous batching (no counterpart in the reference; for
+ * serving loops that let new jobs join running work): each shot runs depth
+ * steps [first_step, min(nz, first_step + steps)) from the wavefields its
+ * rows hold, writes them back (the state the next round continues from) and
+ * adds image rows first_step.. into `image`; shots with no step left have no
+ * work. Shots of different jobs (own fields, image rows, frequency lists and
+ * first steps) share the launch, so jobs admitted between rounds run beside
+ * the ones already running. Workspace, tuning and stream rules as for
+ * wlsq_batch_device_ws (wlsq_batch_workspace_bytes sizes it for any round of
+ * these shots); steps 1..2^24. */
+int wlsq_batch_round_ws(const wlsq_shot *shots, int nshots, int steps, int M,
+                        const float *slowness, const float *omega, float kappa0, float dkappa,
+                        int nk, const wlsq_c32 *table_forw, const wlsq_c32 *table_back,
+                        wlsq_c32 *forw, wlsq_c32 *back, float *image, const wlsq_tuning *tuning,
+                        void *workspace, long long workspace_bytes, void *stream);
 
 /* Reusable plan: the work list (shot groups x frequency groups, longest
  * first) and launch shape are built once and kept on the device. */

@@ -278,7 +278,37 @@ def test_survey_scale_shot_cropped_matches_oracle():
     b.close()
 
 
-def test_served_jobs_at_the_serving_extremes_match_oracle():
+_EXTREME_REFS = {}
+
+
+def _served(kind, models, jobs):
+    """Each job served alone, then all of them at once (max 4 running), by the
+    per-job stream launcher or by the continuous-batching rounds."""
+    from imaging2d_wlsq_amd import serve
+    if kind == "streams":
+        launcher = serve.GpuLauncher(models, slots=4)
+        mk = lambda: serve.Server(launcher, hbm_budget=1 << 34)  # noqa: E731
+    else:
+        launcher = serve.GpuRoundEngine(models, hbm_budget=8 << 30)
+        mk = lambda: serve.RoundServer(launcher, round_points=2e8, max_active=4, lanes=2)  # noqa: E731
+    alone = {}
+    for j in jobs:
+        st = mk().run([j], keep_images=True)
+        alone[j.job_id] = st.images[j.job_id]
+    srv = mk()
+    st = srv.run(jobs, keep_images=True)
+    print(kind, "mix", st.summary(), flush=True)
+    assert st.max_active > 1
+    if kind == "rounds":
+        # jobs joined rounds that others had started: some round holds jobs at different depths
+        assert any(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log)
+        assert launcher.fpool.used == 0 and launcher.ipool.used == 0
+    launcher.close()
+    return alone, st
+
+
+@pytest.mark.parametrize("kind", ["streams", "rounds"])
+def test_served_jobs_at_the_serving_extremes_match_oracle(kind):
     """Jobs of the serving configuration's extremes - apertures 512 and 4096,
     depth counts 500 and 1500 - on two resident 1500 x 4096 models, with few
     frequencies and shots so the oracle stays quick. Each served image (the
@@ -293,7 +323,10 @@ def test_served_jobs_at_the_serving_extremes_match_oracle():
     that round differently drift apart by the same factor: the fast path's
     distance from the oracle is bounded (1e-4) only for jobs whose oracle
     fields stay within 10x of their start, and printed for the others. Both
-    modes give each job the same image alone and in the mix."""
+    modes give each job the same image alone and in the mix. Served by the
+    per-job launcher (one launch per job on its own stream) and by the
+    continuous-batching rounds (jobs of a model advance together, a job
+    admitted later joining rounds the others started)."""
     import torch
     from imaging2d_wlsq_amd import serve, synth
     models = serve.serving_models(n_models=2)
@@ -306,15 +339,8 @@ def test_served_jobs_at_the_serving_extremes_match_oracle():
             x0 = int(rng.integers(0, 4096 - nx + 1)) if rounding == 1 else jobs[i].x0
             jobs.append(serve.Job(len(jobs), i % 2, x0, nx, nz, nf, ns, 0.0, 4400 + i,
                                   rounding=rounding))
-    launcher = serve.GpuLauncher(models, slots=4)
-    alone = {}
-    for j in jobs:
-        st = serve.Server(launcher, hbm_budget=1 << 34).run([j], keep_images=True)
-        alone[j.job_id] = st.images[j.job_id]
-    st = serve.Server(launcher, hbm_budget=1 << 34).run(jobs, keep_images=True)
-    print("mix", st.summary(), flush=True)
-    assert st.max_active > 1
-    refs = {}
+    alone, st = _served(kind, models, jobs)
+    refs = _EXTREME_REFS
     errs = []
     for j in jobs:
         key = (j.model, j.x0, j.nx, j.nz, j.nf, j.nshots, j.seed)
@@ -350,7 +376,6 @@ def test_served_jobs_at_the_serving_extremes_match_oracle():
         elif growth <= 10.0:
             assert e[0] <= 1e-4 and e[1] <= 1e-4, (j, e)
         assert e[2] <= 1e-6, (j, e)
-    launcher.close()
 
 
 def test_graph_keeps_its_table_storage():
@@ -492,3 +517,111 @@ def test_image_sums_from_rows_match_partial_rows(tuning, shared):
     np.testing.assert_array_equal(outs[1][1], outs[0][1])
     np.testing.assert_array_equal(outs[1][2], outs[0][2])
     assert rel(outs[1][0], outs[0][0]) <= 1e-6
+
+
+@pytest.mark.parametrize("stagger", [False, True])
+def test_rounds_match_one_launch(stagger):
+    """wlsq_batch_round_ws: the shots of an uneven batch advanced in rounds of
+    7 depth steps - all from the surface together, or shot k joining at round
+    k beside shots already under way - end with the final fields of one
+    launch over every step bit for bit and its image to the rounding of the
+    atomic order."""
+    import dataclasses
+
+    import torch
+    from imaging2d_wlsq_amd import batch, device
+    p = _small_problem(seed=18, nz=40, nx=800)
+    shots = _uneven_shots(p, 6, 71)
+    sv = batch.Survey(p)
+    t = p.tables
+    b = sv.batch(shots, write_back=True)
+    F0, B0 = b.F.clone(), b.B.clone()
+    ref = b.run().clone()
+    torch.cuda.synchronize()
+    info = b.plan.info()
+    tun = dict(b.tuning, x_per_thread=info["x_per_thread"], shots_per_group=info["shots_per_group"])
+    need = device.batch_workspace_bytes(b.specs, t.M, t.nk, tun)
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    F, B = F0.clone(), B0.clone()
+    img = torch.zeros_like(ref)
+    D = 7
+    r = 0
+    mixed = 0
+    while True:
+        specs = []
+        for k, sp in enumerate(b.specs):
+            start = r - k if stagger else r
+            if start < 0 or start * D >= sp.nz:
+                continue
+            specs.append(dataclasses.replace(sp, first_step=start * D))
+        if not specs:
+            break
+        mixed += len({sp.first_step for sp in specs}) > 1
+        device.batch_round_ws(specs, D, t.M, sv.slowness, b.omega, t.kappa0, t.dkappa, t.nk, sv.tf,
+                              sv.tb, F, B, img, ws, tuning=tun)
+        r += 1
+    torch.cuda.synchronize()
+    print(f"rounds {r}, rounds with shots at different depths {mixed}, plan {info}", flush=True)
+    assert (mixed > 0) == stagger
+    assert torch.equal(F, b.F) and torch.equal(B, b.B)
+    assert rel(img.cpu().numpy(), ref.cpu().numpy()) <= 1e-6
+    b.close()
+
+
+def test_round_entry_refuses_bad_input():
+    """A first_step outside a round, or a round of no steps, is refused before
+    any launch."""
+    import dataclasses
+
+    import torch
+    from imaging2d_wlsq_amd import batch, device
+    from imaging2d_wlsq_amd._lib import WLSQ_EINVAL, WlsqError
+    p = _small_problem(seed=19, nz=20, nx=300)
+    shots = _uneven_shots(p, 2, 72)
+    sv = batch.Survey(p)
+    t = p.tables
+    b = sv.batch(shots, write_back=True)
+    tun = dict(b.tuning)
+    ws = torch.empty(device.batch_workspace_bytes(b.specs, t.M, t.nk, tun), dtype=torch.uint8,
+                     device="cuda")
+    late = [dataclasses.replace(sp, first_step=3) for sp in b.specs]
+    args = (t.M, sv.slowness, b.omega, t.kappa0, t.dkappa, t.nk, sv.tf, sv.tb, b.F, b.B, b.image)
+    for call in (lambda: device.batch_launch_ws(late, *args, ws, tuning=tun),
+                 lambda: device.batch_round_ws(b.specs, 0, *args, ws, tuning=tun),
+                 lambda: device.batch_round_ws([dataclasses.replace(b.specs[0], first_step=-1)], 4,
+                                               *args, ws, tuning=tun),
+                 lambda: device.Plan(late, t.M, tun)):
+        with pytest.raises(WlsqError) as e:
+            call()
+        assert e.value.status == WLSQ_EINVAL
+    b.close()
+
+
+def test_round_server_same_alone_and_in_the_mix():
+    """Continuous batching of a stream of small jobs on two models, at most 3
+    running: every job's image is the same alone and in the mix, jobs joined
+    rounds under way, and the pools are empty at the end."""
+    from imaging2d_wlsq_amd import serve
+    models = [_small_problem(31), _small_problem(32, nz=50, nx=700)]
+    jobs = serve.make_jobs(12, 1e9, 2, 40, 600, seed=19, nx_range=(64, 600), nz_range=(8, 40),
+                           nf_range=(4, 24), shots_range=(1, 9))
+    eng = serve.GpuRoundEngine(models, hbm_budget=1 << 30)
+    alone = {}
+    for j in jobs:
+        alone[j.job_id] = serve.RoundServer(eng, round_points=1e5, min_steps=4).run(
+            [j], keep_images=True).images[j.job_id]
+    for j in jobs:
+        j.arrival = 0.0
+    for lanes in (1, 2):
+        srv = serve.RoundServer(eng, round_points=1e5, min_steps=4, max_active=3, lanes=lanes)
+        st = srv.run(jobs, keep_images=True)
+        print("lanes", lanes, "rounds", srv.rounds_launched, st.summary(), flush=True)
+        assert st.max_active == 3
+        if lanes == 1:
+            assert any(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log)
+        else:
+            assert len({k[-1] for k, _, _, _ in srv.round_log}) == 2
+        for j in jobs:
+            assert rel(st.images[j.job_id], alone[j.job_id]) <= 1e-6, (lanes, j)
+        assert eng.fpool.used == 0 and eng.ipool.used == 0
+    eng.close()

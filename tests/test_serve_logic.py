@@ -120,3 +120,162 @@ def test_footprint_counts_fields_image_and_work_list():
     assert j.footprint() >= fields_image + tables + descriptors
     assert j.footprint(M=25, nk=8192) >= fields_image + 2 * 52 * 8192 * 8
     assert j.points == 1000 * 500 * 100 * 3
+
+
+# ------------------------------------------------------------ continuous batching
+class FakeRoundEngine:
+    """One device that runs queued rounds back to back at `rate` points/s,
+    with `capacity` bytes of pools; records the depth range every round gave
+    every job."""
+
+    def __init__(self, clock, rate=1e9, capacity=1 << 40):
+        self.clock = clock
+        self.rate = rate
+        self.capacity = capacity
+        self.used = 0
+        self.t_free = 0.0
+        self.ranges = {}     # job id -> [(first step, end step), ...]
+        self.rounds = []     # [(cohort, [(job id, first step)], steps)]
+        self.admitted = []
+
+    class S:
+        def __init__(self, job, nbytes):
+            self.job, self.nbytes = job, nbytes
+
+    class H:
+        def __init__(self, t_end):
+            self.t_end = t_end
+
+    def nbytes(self, job):
+        return 2 * job.nshots * job.nf * job.nx * 8 + job.nz * job.nx * 4
+
+    def cohort(self, job):
+        return (job.model,) + serve.job_shape(job) + (job.rounding,)
+
+    def fits(self, job):
+        return self.nbytes(job) <= self.capacity
+
+    def admit(self, job):
+        if self.used + self.nbytes(job) > self.capacity:
+            return None
+        self.used += self.nbytes(job)
+        self.admitted.append(job.job_id)
+        return FakeRoundEngine.S(job, self.nbytes(job))
+
+    def launch_round(self, key, members, steps):
+        pts = 0
+        for s, z in members:
+            j = s.job
+            assert all(key[:len(self.cohort(m.job))] == self.cohort(m.job) for m, _ in members)
+            prev = self.ranges.setdefault(j.job_id, [])
+            assert (prev[-1][1] if prev else 0) == z, "a round must continue where the last ended"
+            end = min(j.nz, z + steps)
+            assert end > z
+            prev.append((z, end))
+            pts += j.nx * j.nf * j.nshots * (end - z)
+        self.rounds.append((key, [(s.job.job_id, z) for s, z in members], steps))
+        start = max(self.clock(), self.t_free)
+        self.t_free = start + pts / self.rate
+        return FakeRoundEngine.H(self.t_free)
+
+    def done(self, h):
+        return self.clock() >= h.t_end
+
+    def image(self, s):
+        return np.full((2, 2), float(s.job.job_id))
+
+    def release(self, s):
+        self.used -= s.nbytes
+
+    def bytes_in_use(self):
+        return self.used
+
+    def sync(self):
+        pass
+
+
+@pytest.mark.parametrize("max_active,cap_jobs,lanes", [(24, 1000, 1), (3, 1000, 1), (24, 2, 1),
+                                                       (24, 1000, 3)])
+def test_round_server_runs_every_step_of_every_job(max_active, cap_jobs, lanes):
+    """Each job's rounds tile its depth range [0, nz) in order; admission is
+    first come first served within the pools and the active-job cap; every job
+    completes with its own image."""
+    jobs = _jobs(n=60, rate=100.0)
+    clock = FakeClock()
+    cap = cap_jobs * max(2 * j.nshots * j.nf * j.nx * 8 + j.nz * j.nx * 4 for j in jobs)
+    eng = FakeRoundEngine(clock, capacity=cap)
+    srv = serve.RoundServer(eng, round_points=2e6, min_steps=4, max_steps=32, max_active=max_active,
+                            lanes=lanes)
+    st = srv.run(jobs, clock=clock, idle=clock.idle, keep_images=True)
+    assert sorted(st.order) == [j.job_id for j in jobs]
+    assert eng.admitted == sorted(eng.admitted)              # first come, first served
+    assert st.max_active <= max_active and st.peak_bytes <= cap and eng.used == 0
+    for j in jobs:
+        r = eng.ranges[j.job_id]
+        assert r[0][0] == 0 and r[-1][1] == j.nz
+        assert all(a[1] == b[0] for a, b in zip(r, r[1:]))
+        assert st.images[j.job_id][0, 0] == j.job_id
+    for _, members, steps in eng.rounds:
+        assert 4 <= steps <= 32 and steps % 4 == 0
+    used = {k[-1] for k, _, _ in eng.rounds}
+    assert used <= set(range(lanes)) and (len(used) > 1) == (lanes > 1)
+    assert st.points == sum(j.points for j in jobs)
+
+
+def test_jobs_join_running_rounds():
+    """Continuous batching: a job that arrives while others run joins the next
+    round of its cohort beside them (a round holds jobs at different depths),
+    and a short job that arrives behind a long one finishes first."""
+    clock = FakeClock()
+    long = serve.Job(0, 0, 0, nx=256, nz=400, nf=16, nshots=1, arrival=0.0, seed=1)
+    short = serve.Job(1, 0, 0, nx=256, nz=20, nf=16, nshots=1, arrival=0.01, seed=2)
+    eng = FakeRoundEngine(clock, rate=1e8)
+    srv = serve.RoundServer(eng, round_points=256 * 16 * 8, min_steps=8, max_steps=8)
+    st = srv.run([long, short], clock=clock, idle=clock.idle)
+    assert st.order == [1, 0]
+    mixed = [m for _, m, _ in eng.rounds if len(m) == 2]
+    assert mixed and all(dict(m)[0] > dict(m)[1] for m in mixed)
+
+
+def test_round_steps_follow_the_round_size():
+    eng = FakeRoundEngine(FakeClock())
+    srv = serve.RoundServer(eng, round_points=1e6, min_steps=16, max_steps=128)
+    mk = lambda nx, nf, ns: serve._RoundJob(serve.Job(0, 0, 0, nx, 100, nf, ns, 0.0, 1), (), None, 0.0)  # noqa: E731
+    assert srv._steps([mk(100, 10, 1)]) == 128        # 1000 points per step: capped
+    assert srv._steps([mk(1000, 100, 1)]) == 16       # 1e5 per step: at least min_steps
+    assert srv._steps([mk(1000, 25, 1)]) == 32        # 2.5e4 per step: 40 steps, cut to 16k
+
+
+def test_job_shape_matches_the_planner_defaults():
+    J = lambda nx, ns, r=0: serve.Job(0, 0, 0, nx, 10, 10, ns, 0.0, 1, rounding=r)  # noqa: E731
+    assert serve.job_shape(J(1024, 2)) == (2, 2) and serve.job_shape(J(1024, 8)) == (2, 4)
+    assert serve.job_shape(J(1025, 2)) == (4, 2) and serve.job_shape(J(2048, 32)) == (4, 2)
+    assert serve.job_shape(J(2049, 2)) == (4, 1) and serve.job_shape(J(512, 1)) == (4, 1)
+    assert serve.job_shape(J(512, 8, r=1)) == (4, 1)
+
+
+def test_range_pool_allocates_disjoint_ranges_and_merges():
+    rng = np.random.default_rng(3)
+    pool = serve.RangePool(10000, align=16)
+    live = {}
+    occupied = np.zeros(10000, bool)
+    for step in range(2000):
+        if live and (rng.random() < 0.45 or len(live) > 40):
+            off = list(live)[int(rng.integers(len(live)))]
+            k = live.pop(off)
+            occupied[off:off + pool._round(k)] = False
+            pool.release(off, k)
+        else:
+            k = int(rng.integers(1, 700))
+            off = pool.alloc(k)
+            if off is None:
+                # first fit failed: no free range holds it
+                assert all(size < pool._round(k) for _, size in pool.free)
+                continue
+            assert off % 16 == 0 and not occupied[off:off + pool._round(k)].any()
+            occupied[off:off + pool._round(k)] = True
+            live[off] = k
+        assert pool.used == int(occupied.sum())
+    for off, k in list(live.items()):
+        pool.release(off, k)
+    assert pool.free == [[0, 10000]] and pool.used == 0
