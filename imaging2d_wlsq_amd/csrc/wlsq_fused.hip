@@ -157,6 +157,8 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         ioff[s] = gp->img_off[s];
     const int img_ld = gp->img_ld;
     const int slow_ld = gp->slow_ld;
+    // image row 0 of the sums from the rows (read once, not per step)
+    float *const imgf_rows = imgf + ioff[0];
     const float *slow0 = a.slow + gp->slow_off;
     // tables have rows of Lp = L + 1 entries (16-byte aligned): taps are
     // gathered in pairs, one 16-byte load per pair and table
@@ -702,9 +704,22 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         // one atomic per (depth, x). Thread t takes x = t, t + NT, ...; the
         // slot of x + NT is that of x plus NT / R (NT is a multiple of R).
         auto flush_rows = [&](const float4 *src, int ld_l) {
-            float *dst = imgf + gp->img_off[0] + (long long)ld_l * img_ld;
+            float *dst = imgf_rows + (long long)ld_l * img_ld;
             const float4 *e0 = src + fps0;
             const unsigned xend = (unsigned)nx * 4u;
+            if (W == 1) {
+                // single-frequency tasks (the wide apertures): one read per x
+                for (unsigned xo = (unsigned)tid * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
+                    const float4 g = *e0;
+                    float v;
+                    if constexpr (EXACT)
+                        v = rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                    else
+                        v = fmaf(g.y, g.w, g.x * g.z);
+                    unsafeAtomicAdd(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
+                }
+                return;
+            }
             for (unsigned xo = (unsigned)tid * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
                 const float4 *e = e0;
                 float4 g = *e;
@@ -752,7 +767,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             // consecutive lanes flush consecutive x: one 32-byte sector of the
             // image carries 8 atomic adds
             for (int s = 0; s < img_rows; ++s) {
-                float *dst = a.img + gp->img_off[s] + (long long)l * img_ld;
+                float *dst = a.img + ioff[s] + (long long)l * img_ld;
                 const float *src = imgrow + (size_t)s * W * nxr;
                 for (int x = tid; x < nx; x += NT) {
                     float v = src[x];
