@@ -254,7 +254,8 @@ def probe_fma(log):
 
 def bench_c3(args, log):
     """128 shots of uneven aperture / frequency count on an 800 x 2048 model in
-    one launch, replayed as a hipGraph; launch shape autotuned on 16 shots."""
+    one launch, replayed as a hipGraph; launch shape autotuned on the first
+    --c3-tune-shots shots (all 128 by default)."""
     import torch
 
     from imaging2d_wlsq_amd import batch, synth
@@ -264,8 +265,11 @@ def bench_c3(args, log):
     sv = batch.Survey(p)
     tuning = None
     if not args.no_tune:
-        probe = sv.batch(p.shots[:16], tune=True, reps=1)
+        t_tune = time.perf_counter()
+        probe = sv.batch(p.shots[:args.c3_tune_shots], tune=True, reps=1)
         tuning = probe.tuning
+        log(f"c3: tuned on {min(args.c3_tune_shots, len(p.shots))} shots in "
+            f"{time.perf_counter() - t_tune:.1f} s: {tuning}")
         probe.close()
         del probe
     b = sv.batch(p.shots, tuning=tuning, graph=True)
@@ -415,6 +419,8 @@ def main():
                     help="main leg: autotune the launch shape (default: the planner's shape, which "
                          "the committed profile covers)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
+    ap.add_argument("--c3-tune-shots", type=int, default=128,
+                    help="shots of the uneven batch the launch shape is tuned on")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the serving run (0: skip; the configuration is 2000)")
     ap.add_argument("--serve-slots", type=int, default=8, help="streams of --serve-mode streams")

@@ -42,7 +42,7 @@ class Tuning(Structure):
                 ("x_per_thread", c_int), ("lds_budget", c_int), ("tables_conjugate", c_int),
                 ("image_mode", c_int), ("tables_symmetric", c_int), ("tap_loads", c_int),
                 ("strip_gathers", c_int), ("double_rows", c_int), ("depth_steps", c_int),
-                ("image_planes", c_int), ("rounding", c_int)]
+                ("image_planes", c_int), ("rounding", c_int), ("strips_per_thread", c_int)]
 
 
 class Shot(Structure):

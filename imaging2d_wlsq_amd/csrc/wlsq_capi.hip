@@ -401,10 +401,18 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     // other shapes run up to 512.
     const int max_block = ((R == 4 && S == 1) || (R == 2 && S == 2)) ? 1024 : 512;
     const int max_strips = (max_nx + R - 1) / R;
-    // strips per thread: 2 when the aperture has more strips than the
-    // workgroup has threads (8 x per thread, or 4 with 1024 threads)
+    // strips per thread: as tuned, else 2 when the aperture has more strips
+    // than the workgroup has threads (8 x per thread, or 4 with 1024 threads)
     const int threads = t.block > 0 ? t.block : max_block;
-    const int C = max_strips <= threads ? 1 : 2;
+    if (t.strips_per_thread < 0 || t.strips_per_thread > 2)
+        return fail(WLSQ_EINVAL, "strips_per_thread must be 0, 1 or 2 (got %d)", t.strips_per_thread);
+    const int C = t.strips_per_thread > 0 ? t.strips_per_thread : (max_strips <= threads ? 1 : 2);
+    if (C == 2 && !((R == 4 && S == 1) || R == 8))
+        return fail(WLSQ_EUNSUPPORTED, "two strips per thread are compiled for 4 x per thread with one "
+                                       "shot per group and for 8 x per thread");
+    if (C == 1 && max_strips > threads && t.strips_per_thread == 1)
+        return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips: above %d threads with one strip "
+                                       "per thread", max_nx, max_strips, threads);
     if (max_strips > (R == 2 ? 1 : C) * max_block || (C == 2 && R == 2) || (C == 2 && R == 4 && S != 1))
         return fail(WLSQ_EUNSUPPORTED, "aperture nx=%d needs %d strips of %d x (max %d)", max_nx,
                     max_strips, R, (R == 2 || S != 1 ? 1 : 2) * max_block);

@@ -183,6 +183,15 @@ typedef struct {
                              shot per group and more than 512 threads (the
                              planner picks them when those fields are 0);
                              tables_symmetric is ignored. */
+    int strips_per_thread; /* 0 = auto (two only where the aperture has more
+                              strips than the workgroup has threads), 1, or 2:
+                              each thread steps two strips of its task's rows
+                              (4 x per thread with one shot per group and more
+                              than 512 threads, or 8 x per thread), so a task
+                              holds up to twice the frequency rows, in one copy
+                              of the rows (two barriers per step). With
+                              lds_budget, the autotuner uses it to trade rows
+                              per task against tasks. */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,

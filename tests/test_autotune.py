@@ -23,6 +23,12 @@ def test_candidates_cover_the_planner_shapes_within_limits():
         assert not (c["x_per_thread"] == 2 and c["shots_per_group"] == 1)
         assert not (c["x_per_thread"] == 8 and c["shots_per_group"] != 1)
     assert not _has(autotune.candidates(False, False, 1024), tables_symmetric=1)
+    # two strips per thread where one would do: 1024 threads, one shot, 4 x
+    assert _has(narrow, strips_per_thread=2, x_per_thread=4, shots_per_group=1, block=1024)
+    assert not _has(wide, strips_per_thread=2)  # 8192 wide: the planner's own choice
+    for c in narrow:
+        if c.get("strips_per_thread") == 2:
+            assert c["x_per_thread"] == 4 and c["shots_per_group"] == 1 and c["lds_budget"] > 0
 
 
 @pytest.mark.gpu

@@ -625,3 +625,37 @@ def test_round_server_same_alone_and_in_the_mix():
             assert rel(st.images[j.job_id], alone[j.job_id]) <= 1e-6, (lanes, j)
         assert eng.fpool.used == 0 and eng.ipool.used == 0
     eng.close()
+
+
+@pytest.mark.parametrize("budget", [0, 120 << 10])
+def test_two_strips_per_thread_match_one(budget):
+    """strips_per_thread = 2 on apertures one strip per thread covers (twice
+    the frequency rows per task, one copy of the rows): the final fields of
+    the one-strip shape bit for bit and its image to the rounding of the
+    atomic order; invalid requests are refused."""
+    import torch
+    from imaging2d_wlsq_amd import batch, device
+    from imaging2d_wlsq_amd._lib import WLSQ_EINVAL, WLSQ_EUNSUPPORTED, WlsqError
+    p = _small_problem(seed=20, nz=30, nx=900)
+    shots = _uneven_shots(p, 6, 91)
+    sv = batch.Survey(p)
+    base = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym), x_per_thread=4,
+                shots_per_group=1, block=1024)
+    outs = []
+    for t in (base, dict(base, strips_per_thread=2, lds_budget=budget)):
+        b = sv.batch(shots, tuning=t, write_back=True)
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), b.F.cpu().numpy(), b.B.cpu().numpy(), b.plan.info()))
+        print("tuning", t, b.plan.info(), flush=True)
+        specs = b.specs
+        b.close()
+    assert outs[1][3]["items"] <= outs[0][3]["items"]
+    np.testing.assert_array_equal(outs[1][1], outs[0][1])
+    np.testing.assert_array_equal(outs[1][2], outs[0][2])
+    assert rel(outs[1][0], outs[0][0]) <= 1e-6
+    for t, code in ((dict(base, strips_per_thread=3), WLSQ_EINVAL),
+                    (dict(base, strips_per_thread=2, shots_per_group=2, block=512), WLSQ_EUNSUPPORTED)):
+        with pytest.raises(WlsqError) as e:
+            device.Plan(specs, p.tables.M, t)
+        assert e.value.status == code
