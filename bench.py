@@ -264,7 +264,10 @@ def bench_c3(args, log):
     p = synth.config_c3()
     sv = batch.Survey(p)
     tuning = None
-    if not args.no_tune:
+    if args.c3_tuning:
+        tuning = dict(json.loads(args.c3_tuning), tables_conjugate=int(sv.conj),
+                      tables_symmetric=int(sv.sym))
+    elif not args.no_tune:
         t_tune = time.perf_counter()
         probe = sv.batch(p.shots[:args.c3_tune_shots], tune=True, reps=1)
         tuning = probe.tuning
@@ -351,7 +354,9 @@ def bench_serving(args, log):
     budget = 32 << 30
     rounds = args.serve_mode == "rounds"
     if rounds:
-        launcher = serve.GpuRoundEngine(models, hbm_budget=budget)
+        extra = {"strips_per_thread": args.serve_strips, "lds_budget": args.serve_lds_budget}
+        launcher = serve.GpuRoundEngine(models, hbm_budget=budget, one_shot_tuning=extra)
+        launcher.warm(lanes=args.serve_lanes)
         mk = lambda: serve.RoundServer(launcher, round_points=args.serve_round_points,  # noqa: E731
                                        max_active=args.serve_max_active, lanes=args.serve_lanes,
                                        min_steps=args.serve_min_steps)
@@ -383,7 +388,8 @@ def bench_serving(args, log):
                   "rounds_with_jobs_at_different_depths":
                       sum(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log),
                   "round_points": args.serve_round_points, "max_active": args.serve_max_active,
-                  "lanes": args.serve_lanes, "min_steps": args.serve_min_steps})
+                  "lanes": args.serve_lanes, "min_steps": args.serve_min_steps,
+                  "one_shot_tuning": launcher.one_shot_tuning})
     else:
         s.update({"mode": "one launch per job on its own stream", "slots": args.serve_slots})
     s.update({"arrival_rate_jobs_per_s": rate, "offered_load": args.serve_load,
@@ -421,16 +427,23 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
     ap.add_argument("--c3-tune-shots", type=int, default=128,
                     help="shots of the uneven batch the launch shape is tuned on")
+    ap.add_argument("--c3-tuning", type=str, default="",
+                    help="launch shape of the uneven batch as JSON (no tuning run), e.g. the "
+                         "tuner's choice, for profiler passes")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the serving run (0: skip; the configuration is 2000)")
     ap.add_argument("--serve-slots", type=int, default=8, help="streams of --serve-mode streams")
     ap.add_argument("--serve-mode", choices=["rounds", "streams"], default="rounds")
     ap.add_argument("--serve-round-points", type=float, default=3e9,
                     help="points per round of --serve-mode rounds")
-    ap.add_argument("--serve-max-active", type=int, default=24,
+    ap.add_argument("--serve-max-active", type=int, default=8,
                     help="running jobs at most (--serve-mode rounds)")
     ap.add_argument("--serve-lanes", type=int, default=4,
                     help="round sequences per cohort (--serve-mode rounds)")
+    ap.add_argument("--serve-strips", type=int, default=2, choices=[1, 2],
+                    help="strips per thread of the one-shot cohorts (--serve-mode rounds)")
+    ap.add_argument("--serve-lds-budget", type=int, default=160 << 10,
+                    help="LDS budget of the one-shot cohorts (--serve-mode rounds)")
     ap.add_argument("--serve-min-steps", type=int, default=64,
                     help="depth steps per round at least (--serve-mode rounds)")
     ap.add_argument("--c5-shots", type=int, default=125,

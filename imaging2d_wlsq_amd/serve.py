@@ -188,7 +188,7 @@ class GpuLauncher:
         ws = self._ws[slot]
         if ws is None or ws.numel() < need:
             with torch.cuda.stream(stream):
-                ws = torch.empty(max(need, 4 << 20), dtype=torch.uint8, device=self.dev)
+                ws = torch.empty(max(need, 16 << 20), dtype=torch.uint8, device=self.dev)
             self._ws[slot] = ws
         device.batch_launch_ws(specs, t.M, sv.slowness, omega, t.kappa0, t.dkappa, t.nk, sv.tf,
                                sv.tb, F, B, image, ws, tuning=tun, stream=stream)
@@ -355,7 +355,7 @@ class RoundServer:
     run and the engine can place their wavefields and image."""
 
     def __init__(self, engine, round_points: float = 3e9, min_steps: int = 16,
-                 max_steps: int = 512, depth: int = 2, max_active: int = 24, lanes: int = 1):
+                 max_steps: int = 512, depth: int = 2, max_active: int = 8, lanes: int = 1):
         self.engine = engine
         # jobs of one cohort are spread over `lanes` round sequences (the lane
         # with the least work left takes a new job); the lanes' rounds are
@@ -442,7 +442,7 @@ class RoundServer:
             for key, members in cohorts.items():
                 q = inflight[key]
                 while len(q) < self.depth:
-                    run = [r for r in members if r.z < r.job.nz]
+                    run = [r for r in members if r.z < r.job.nz and eng.ready(r.state)]
                     if not run:
                         break
                     steps = self._steps(run)
@@ -487,10 +487,17 @@ class GpuRoundEngine:
     round is one ``wlsq_batch_round_ws`` launch."""
 
     def __init__(self, models: Sequence[synth.Problem], hbm_budget: int = 32 << 30, dev="cuda",
-                 image_share: float = 0.1, omega_slots: int = 1 << 20):
+                 image_share: float = 0.1, omega_slots: int = 1 << 20,
+                 one_shot_tuning: Optional[dict] = None):
         self.surveys = [batch.Survey(p, dev) for p in models]
         self.dev = torch.device(dev)
         self.M = models[0].tables.M
+        # extra tuning of the cohorts with one shot per group (fast path):
+        # by default two strips per thread with the whole LDS, so a task holds
+        # two frequency rows of apertures up to about 5000 (measured on the
+        # 2000-job stream: 6-7 % above one strip per thread)
+        self.one_shot_tuning = dict(strips_per_thread=2, lds_budget=160 << 10) \
+            if one_shot_tuning is None else dict(one_shot_tuning)
         # table families: the first model of each family holds its tables
         self.family: List[int] = []
         for i, p in enumerate(models):
@@ -525,6 +532,9 @@ class GpuRoundEngine:
         self.streams: Dict[tuple, torch.cuda.Stream] = {}
         self.ws: Dict[tuple, List[Optional[torch.Tensor]]] = {}
         self.flip: Dict[tuple, int] = {}
+        # inputs of admitted jobs are generated here; a job joins rounds once
+        # they are ready, so running jobs never wait for a new job's inputs
+        self.gen_stream = torch.cuda.Stream(device=self.dev)
 
     def cohort(self, job: Job) -> tuple:
         return (self.family[job.model],) + job_shape(job) + (job.rounding,)
@@ -559,21 +569,51 @@ class GpuRoundEngine:
             return None
         p = self.surveys[job.model].problem
         shots = job.shots()
-        key = self.cohort(job)
-        stream = self._stream(key)
-        with torch.cuda.stream(stream):
+        self._stream(self.cohort(job))
+        with torch.cuda.stream(self.gen_stream):
             # F of the job's shots in the first half of its range, B in the second
             _, _, offs = synth.device_fields(p, shots, device=self.dev,
                                              out_f=self.fields[f_off:f_off + n],
                                              out_b=self.fields[f_off + n:f_off + 2 * n])
             self.images[i_off:i_off + job.nz * job.nx].zero_()
             self.omegas[o_off:o_off + job.nf].copy_(torch.from_numpy(shots[0].omega))
+        ready = torch.cuda.Event()
+        ready.record(self.gen_stream)
         specs = [device.ShotSpec(f_off=f_off + o, b_off=f_off + n + o, img_off=i_off,
                                  slow_off=self.slow_base[job.model] + job.x0, ld=job.nx,
                                  img_ld=job.nx, slow_ld=p.model.nx,
                                  nx=job.nx, nz=job.nz, nf=job.nf, omega_off=o_off) for o in offs]
-        return _PoolJob(job, f_off, 2 * n, i_off, job.nz * job.nx, o_off, specs,
-                        8 * 2 * n + 4 * job.nz * job.nx)
+        st = _PoolJob(job, f_off, 2 * n, i_off, job.nz * job.nx, o_off, specs,
+                      8 * 2 * n + 4 * job.nz * job.nx)
+        st.ready = ready
+        return st
+
+    def ready(self, state) -> bool:
+        """The job's inputs are on the device (it may join a round)."""
+        return state.ready.query()
+
+    def warm(self, lanes: int = 1, shapes=((2, 2), (2, 4), (4, 2), (4, 1))) -> None:
+        """One one-step round of a tiny shot per table family, kernel shape
+        and lane, so each kernel variant is loaded and each lane's stream and
+        workspaces exist before the first job (a server's start-up, not part
+        of its service)."""
+        for fam in sorted(set(self.family)):
+            p = self.surveys[fam].problem
+            for R, S in shapes:
+                f_off, i_off, o_off = self.fpool.alloc(128), self.ipool.alloc(64), self.opool.alloc(1)
+                spec = device.ShotSpec(f_off=f_off, b_off=f_off + 64, img_off=i_off,
+                                       slow_off=self.slow_base[fam], ld=64, img_ld=64,
+                                       slow_ld=p.model.nx, nx=64, nz=1, nf=1, omega_off=o_off)
+                self.fields[f_off:f_off + 128].zero_()
+                self.omegas[o_off:o_off + 1].fill_(10.0)
+                torch.cuda.synchronize()
+                for lane in range(max(1, lanes)):
+                    for _ in range(2):  # both workspaces of the lane
+                        self._launch((fam, R, S, 0, lane), [spec], 1)
+                torch.cuda.synchronize()
+                self.fpool.release(f_off, 128)
+                self.ipool.release(i_off, 64)
+                self.opool.release(o_off, 1)
 
     def _tuning(self, key) -> dict:
         fam, R, S, rounding = key[:4]
@@ -582,6 +622,8 @@ class GpuRoundEngine:
                  shots_per_group=S)
         if rounding:
             t["rounding"] = 1
+        elif S == 1:
+            t.update(self.one_shot_tuning)
         return t
 
     def _ws_bytes(self, specs, nk) -> int:
@@ -594,14 +636,17 @@ class GpuRoundEngine:
                 2 * al(8 * (L + 1) * nk) + 256)
 
     def launch_round(self, key, members, steps):
-        fam = key[0]
-        sv = self.surveys[fam]
-        t = sv.problem.tables
         specs = []
         for state, z in members:
             for sp in state.specs:
                 sp.first_step = z
                 specs.append(sp)
+        return self._launch(key, specs, steps)
+
+    def _launch(self, key, specs, steps):
+        fam = key[0]
+        sv = self.surveys[fam]
+        t = sv.problem.tables
         stream = self._stream(key)
         k = self.flip[key]
         self.flip[key] = k ^ 1

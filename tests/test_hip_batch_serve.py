@@ -290,6 +290,7 @@ def _served(kind, models, jobs):
         mk = lambda: serve.Server(launcher, hbm_budget=1 << 34)  # noqa: E731
     else:
         launcher = serve.GpuRoundEngine(models, hbm_budget=8 << 30)
+        launcher.warm(lanes=2)
         mk = lambda: serve.RoundServer(launcher, round_points=2e8, max_active=4, lanes=2)  # noqa: E731
     alone = {}
     for j in jobs:

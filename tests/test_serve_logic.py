@@ -181,6 +181,9 @@ class FakeRoundEngine:
     def done(self, h):
         return self.clock() >= h.t_end
 
+    def ready(self, s):
+        return True
+
     def image(self, s):
         return np.full((2, 2), float(s.job.job_id))
 
