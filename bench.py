@@ -424,6 +424,8 @@ def main():
     ap.add_argument("--tune-c2", action="store_true",
                     help="main leg: autotune the launch shape (default: the planner's shape, which "
                          "the committed profile covers)")
+    ap.add_argument("--exact-steps", type=int, default=5,
+                    help="timed launches of the main workload with exact image sums (0: skip)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the uneven batch (0: skip)")
     ap.add_argument("--c3-tune-shots", type=int, default=128,
                     help="shots of the uneven batch the launch shape is tuned on")
@@ -553,6 +555,35 @@ def main():
     log(f"fused: {ms_per_step:.2f} ms/step, kernel {k_avg:.2f} ms, {value:.3e} points/s, "
         f"{achieved_tflops:.1f} TFLOP/s")
 
+    # the same launches with exact image sums (bitwise-repeatable images): their time and image
+    exact = None
+    if args.exact_steps > 0 and conj:
+        xplan = device.Plan(specs, t.M, dict(tuning, exact_image=1))
+        ximgs = []
+        xms = []
+        for k in range(args.exact_steps + 1):
+            image.zero_()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            xplan.launch(slow, omega, t.kappa0, t.dkappa, t.nk, tf, tb, F, B, image, write_back=False,
+                         stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if k > 0:
+                xms.append(e0.elapsed_time(e1))
+            ximgs.append(image.clone())
+        xplan.close()
+        x_avg = sum(xms) / len(xms)
+        rep = all(torch.equal(ximgs[0].view(torch.int32), x.view(torch.int32)) for x in ximgs[1:])
+        xrel = float((ximgs[0].double() - img_fused.double()).norm() / img_fused.double().norm())
+        exact = {"what": "exact_image = 1: task sums added as 64-bit integers on the 2^-149 grid, "
+                         "rounded once (fill, fused kernel, finish kernel per launch)",
+                 "ms_per_launch": x_avg, "vs_float_atomics": x_avg / k_avg,
+                 "launches_bitwise_equal": rep, "launches": len(ximgs),
+                 "image_rel_l2_vs_float_atomics": xrel}
+        log(f"exact image sums: {x_avg:.2f} ms per launch ({x_avg / k_avg:.3f} x), "
+            f"{len(ximgs)} launches bitwise equal: {rep}, rel L2 vs float atomics {xrel:.2e}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -583,6 +614,8 @@ def main():
 
     if probe is not None:
         out["probe_fma"] = probe
+    if exact is not None:
+        out["exact_image"] = exact
 
     # ---------------------------------------------------------------- plain
     if not args.no_plain:

@@ -28,6 +28,7 @@ EXPORTS = [
     "wlsq_graph_create", "wlsq_graph_launch", "wlsq_graph_destroy", "wlsq_batch_points",
     "c64fft1dforw", "c64fft1dback", "wlsq_trace_spectra_device", "wlsq_ktable_device_noalloc",
     "wlsq_batch_workspace_bytes", "wlsq_batch_device_ws", "wlsq_probe_fma", "wlsq_batch_round_ws",
+    "wlsq_probe_exact_sum",
 ]
 
 
@@ -42,7 +43,8 @@ class Tuning(Structure):
                 ("x_per_thread", c_int), ("lds_budget", c_int), ("tables_conjugate", c_int),
                 ("image_mode", c_int), ("tables_symmetric", c_int), ("tap_loads", c_int),
                 ("strip_gathers", c_int), ("double_rows", c_int), ("depth_steps", c_int),
-                ("image_planes", c_int), ("rounding", c_int), ("strips_per_thread", c_int)]
+                ("image_planes", c_int), ("rounding", c_int), ("strips_per_thread", c_int),
+                ("exact_image", c_int)]
 
 
 class Shot(Structure):
@@ -116,6 +118,8 @@ def lib():
     L.wlsq_graph_destroy.argtypes = [vp]
     L.wlsq_probe_fma.restype = ci
     L.wlsq_probe_fma.argtypes = [ci, ci, vp, vp]
+    L.wlsq_probe_exact_sum.restype = ci
+    L.wlsq_probe_exact_sum.argtypes = [vp, c_longlong, ci, vp, vp]
     L.wlsq_batch_points.restype = c_longlong
     L.wlsq_batch_points.argtypes = [POINTER(Shot), ci]
     # transforms: cpp_src/interface.py:36-45 binds the MKL pair the same way

@@ -254,6 +254,8 @@ struct PlanImpl {
     int nplanes = 1;             // image planes of the direct atomics
     long long img_extent = 0;    // floats of the image the shots touch
     DevBuf planes;
+    int exact_image = 0;         // image sums accumulated exactly, rounded once after the launch
+    DevBuf xacc;                 // their accumulators: kExactPlanes planes of img_extent words
     int strip_gathers = 1;
     int double_rows = 0;
     int depth_steps = 0;         // depth steps per launch (0: all)
@@ -281,6 +283,8 @@ struct PlanImpl {
             (void)hipFree(fields.p);
         if (planes.p)
             (void)hipFree(planes.p);
+        if (xacc.p)
+            (void)hipFree(xacc.p);
     }
     // table copy storage for nk samples (allocates: call outside graph capture).
     // A graph of this plan holds d_tt: it is never reallocated while one lives.
@@ -609,6 +613,20 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         }
         P->nplanes = K >= 2 ? K : 1;
     }
+    // Exact image sums: integer accumulation of each task's flushed sums (the
+    // flushes of image modes 1 and 4), rounded once after the launch.
+    if (t.exact_image < 0 || t.exact_image > 1)
+        return fail(WLSQ_EINVAL, "exact_image must be 0 or 1 (got %d)", t.exact_image);
+    if (t.exact_image) {
+        if (P->img_lds == 0)
+            return fail(WLSQ_EUNSUPPORTED, "exact_image needs image mode 1 or 4 (the task sums a "
+                                           "flush adds); these shapes add per point");
+        if (!P->shape.conj)
+            return fail(WLSQ_EUNSUPPORTED, "exact_image is compiled for conjugate tables "
+                                           "(tables_conjugate = 1)");
+        P->exact_image = 1;
+        P->nplanes = 1;
+    }
     if (upload && !P->groups.empty()) {
         HIP_TRY(hipMalloc(&P->d_groups, sizeof(Group) * P->groups.size()));
         HIP_TRY(hipMemcpy(P->d_groups, P->groups.data(), sizeof(Group) * P->groups.size(),
@@ -678,6 +696,13 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
         a.nplanes = P.nplanes;
         HIP_TRY(launch_fill_f32(a.planes, (long long)P.nplanes * P.img_extent, 0.f, st));
     }
+    if (P.exact_image) {
+        // zeroed by a fill kernel (no memset node when captured in a graph)
+        WLSQ_TRY(P.xacc.reserve(sizeof(unsigned long long) * kExactPlanes * (size_t)P.img_extent));
+        a.xacc = P.xacc.as<unsigned long long>();
+        a.xstride = P.img_extent;
+        HIP_TRY(launch_fill_f32(P.xacc.as<float>(), 2ll * kExactPlanes * P.img_extent, 0.f, st));
+    }
     const int ds = P.depth_steps;
     if (ds == 0 || ds >= P.max_nz) {
         a.l0 = 0;
@@ -685,6 +710,8 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
         HIP_TRY(launch_fused(P.shape, (int)P.items.size(), P.lds, st, a));
         if (P.nplanes > 1)
             HIP_TRY(launch_planes_reduce(a.planes, a.plane_stride, a.nplanes, image, P.img_extent, st));
+        if (P.exact_image)
+            HIP_TRY(launch_exact_finish(a.xacc, a.xstride, P.img_extent, image, st));
         return WLSQ_OK;
     }
     // Depth range split over launches of ds steps: each launch starts from
@@ -707,6 +734,8 @@ int launch_plan(PlanImpl &P, const float *slowness, const float *omega, float ka
     }
     if (P.nplanes > 1)
         HIP_TRY(launch_planes_reduce(a.planes, a.plane_stride, a.nplanes, image, P.img_extent, st));
+    if (P.exact_image)
+        HIP_TRY(launch_exact_finish(a.xacc, a.xstride, P.img_extent, image, st));
     return WLSQ_OK;
 }
 
@@ -860,6 +889,9 @@ int workspace_tuning(const wlsq_tuning *tuning, wlsq_tuning *t)
     if (t->depth_steps != 0)
         return fail(WLSQ_EINVAL, "launches into a workspace run every depth step in one launch "
                                  "(depth_steps must be 0, got %d)", t->depth_steps);
+    if (t->exact_image != 0)
+        return fail(WLSQ_EINVAL, "exact_image needs a plan's accumulators (use a plan, "
+                                 "wlsq_batch_device or wlsq_ktable_device)");
     t->image_planes = 1;
     return WLSQ_OK;
 }
@@ -1274,6 +1306,24 @@ int wlsq_probe_fma(int blocks, int iters, float *out, void *stream)
     return WLSQ_OK;
 }
 
+int wlsq_probe_exact_sum(const float *terms, long long n, int m, float *out, void *stream)
+{
+    if (!out || (n > 0 && !terms) || n < 0 || m < 0 || (m > 0 && n > (1ll << 24)))
+        return fail(WLSQ_EINVAL, "exact sum probe: invalid arguments (n=%lld m=%d)", n, m);
+    if (m == 0)
+        return WLSQ_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned long long *acc = nullptr;
+    HIP_TRY(hipMalloc(&acc, sizeof(unsigned long long) * kExactPlanes * (size_t)m));
+    hipError_t e = launch_exact_probe(terms, n, m, acc, out, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    (void)hipFree(acc);
+    if (e != hipSuccess)
+        return fail(WLSQ_EHIP, "exact sum probe: %s", hipGetErrorString(e));
+    return WLSQ_OK;
+}
+
 long long wlsq_batch_points(const wlsq_shot *shots, int nshots)
 {
     long long p = 0;
@@ -1370,6 +1420,8 @@ int wlsq_graph_create(wlsq_graph *out, wlsq_plan plan, const float *slowness, co
             WLSQ_TRY(P.fields.reserve(sizeof(float2) * (size_t)(P.f_extent + P.b_extent)));
         if (P.nplanes > 1)
             WLSQ_TRY(P.planes.reserve(sizeof(float) * (size_t)P.nplanes * (size_t)P.img_extent));
+        if (P.exact_image)
+            WLSQ_TRY(P.xacc.reserve(sizeof(unsigned long long) * kExactPlanes * (size_t)P.img_extent));
     }
     hipStream_t cap = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));

@@ -53,7 +53,7 @@ __device__ __forceinline__ void wave_sync()
 // at sub-row (p mod R), slot p / R (swz): lanes that read p = R*lane + c for a
 // fixed c touch consecutive 16-byte slots.
 template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP,
-          bool EXACT>
+          bool EXACT, bool XACC>
 __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 {
     static_assert(!(SYM && EXACT), "the CPU path's rounding sums the taps in order");
@@ -159,6 +159,15 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     const int slow_ld = gp->slow_ld;
     // image row 0 of the sums from the rows (read once, not per step)
     float *const imgf_rows = imgf + ioff[0];
+    // a task's image sums: float atomics, or (XACC, compiled apart so the
+    // float-atomic kernels are untouched) exact integer accumulation into
+    // a.xacc, after which the image does not depend on the order of the adds
+    auto img_add = [&](float *p, float v) {
+        if constexpr (XACC)
+            exact_add(a.xacc, a.xstride, (long long)(p - a.img), v);
+        else
+            unsafeAtomicAdd(p, v);
+    };
     const float *slow0 = a.slow + gp->slow_off;
     // tables have rows of Lp = L + 1 entries (16-byte aligned): taps are
     // gathered in pairs, one 16-byte load per pair and table
@@ -716,7 +725,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                         v = rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
                     else
                         v = fmaf(g.y, g.w, g.x * g.z);
-                    unsafeAtomicAdd(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
+                    img_add(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
                 }
                 return;
             }
@@ -736,7 +745,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                     else
                         v = fmaf(g.y, g.w, fmaf(g.x, g.z, v));
                 }
-                unsafeAtomicAdd(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
+                img_add(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
             }
         };
         using c0 = std::integral_constant<int, 0>;
@@ -773,7 +782,7 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                     float v = src[x];
                     for (int w = 1; w < W; ++w)
                         v += src[(size_t)w * nxr + x];
-                    unsafeAtomicAdd(dst + x, v);
+                    img_add(dst + x, v);
                 }
             }
         }
@@ -807,15 +816,33 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
 }
 
 template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP,
-          bool EXACT>
-hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
+          bool EXACT, bool XACC>
+hipError_t launch_fused_k(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
 {
-    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP, EXACT>;
+    auto k = fused_kernel<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP, EXACT, XACC>;
     hipError_t e = ensure_lds(k, lds);
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k, dim3(n_items), dim3(block), lds, st, a);
     return hipGetLastError();
+}
+
+// exact image sums (a.xacc) are compiled for the shapes that flush task sums
+// (no wave-private transpose) with conjugate tables
+template <int L_, int R, int S, bool CONJ, bool SYM, bool PAIRED, int C, int MAXT, bool XP,
+          bool EXACT>
+hipError_t launch_fused_t(int n_items, int block, size_t lds, hipStream_t st, const FusedArgs &a)
+{
+    if constexpr (!XP && CONJ) {
+        if (a.xacc)
+            return launch_fused_k<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP, EXACT, true>(n_items, block,
+                                                                                       lds, st, a);
+    } else {
+        if (a.xacc)
+            return hipErrorInvalidValue;
+    }
+    return launch_fused_k<L_, R, S, CONJ, SYM, PAIRED, C, MAXT, XP, EXACT, false>(n_items, block, lds,
+                                                                                  st, a);
 }
 
 template <int L_, bool P, int R, int S, int C, int MAXT, bool XP = false>

@@ -17,6 +17,10 @@ constexpr int kMaxKappaSamples = 1 << 22;
 constexpr int kMaxImagePlanes = 16;
 // Most depth steps of one round of a continuous batch (wlsq_batch_round_ws).
 constexpr int kMaxRoundSteps = 1 << 24;
+// Exact image accumulation: 64-bit words per image point (16 digit words of
+// 16-bit spacing from 2^-149, then a flags word).
+constexpr int kExactWords = 16;
+constexpr int kExactPlanes = kExactWords + 1;
 
 // One shot group: up to 4 shots that share aperture, depth count and
 // frequency list, so one coefficient load serves all of them.
@@ -72,6 +76,8 @@ struct FusedArgs {
     int double_rows;    // 1: two copies of the rows in LDS, one barrier per step
     int l0, l1;         // depth steps [z0 + l0, min(nz, z0 + l1)) of each task in this
                         // launch (z0 of the task's group)
+    unsigned long long *xacc;  // non-null: the image sums go into these exact accumulators
+    long long xstride;         // (kExactPlanes planes laid out like img, `xstride` apart)
 };
 
 // Launch shape of the fused kernel; chosen by the planner.
@@ -138,6 +144,14 @@ hipError_t launch_planes_reduce(const float *planes, long long stride, int nplan
                                 long long n, hipStream_t st);
 // compact (nk, L) table -> rows of L + 1 entries (last 0), read by the fused kernel
 hipError_t launch_table_pad(const float2 *in, float2 *out, int nk, int L, hipStream_t st);
+// img[i] += the float nearest the exact sum in acc (words `stride` apart), for
+// i < n where any term was added
+hipError_t launch_exact_finish(const unsigned long long *acc, long long stride, long long n,
+                               float *img, hipStream_t st);
+// out[j] = the float nearest sum_i terms[i * m + j] (i < n), the terms added by
+// many threads in no fixed order into zeroed accumulators acc (kExactPlanes * m words)
+hipError_t launch_exact_probe(const float *terms, long long n, int m, unsigned long long *acc,
+                              float *out, hipStream_t st);
 
 // Records the message returned by wlsq_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);

@@ -462,6 +462,84 @@ hipError_t launch_probe_fma(int blocks, int iters, float *out, hipStream_t st)
     return hipGetLastError();
 }
 
+// Exact image sums: img[i] += the float nearest the sum accumulated for point i
+// (points no term reached keep their value).
+__global__ void __launch_bounds__(256) exact_finish_kernel(const unsigned long long *__restrict__ acc,
+                                                           long long stride, long long n,
+                                                           float *__restrict__ img)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        unsigned long long w[kExactWords];
+        unsigned long long any = 0;
+#pragma unroll
+        for (int k = 0; k < kExactWords; ++k) {
+            w[k] = acc[k * stride + i];
+            any |= w[k];
+        }
+        const unsigned long long f = acc[kExactWords * stride + i];
+        if (any | f)
+            img[i] += exact_round(w, f);
+    }
+}
+
+// Measurement aid: thread t of block b adds terms[i * m + j] for the i of its
+// stride, so the terms of one sum arrive in no fixed order; then one thread
+// per sum rounds it.
+__global__ void __launch_bounds__(256) exact_probe_add_kernel(const float *__restrict__ terms,
+                                                              long long n, int m,
+                                                              unsigned long long *acc)
+{
+    const long long total = n * m;
+    for (long long e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (long long)gridDim.x * 256)
+        exact_add(acc, m, e % m, terms[e]);
+}
+
+__global__ void __launch_bounds__(256) exact_probe_round_kernel(const unsigned long long *__restrict__ acc,
+                                                                int m, float *__restrict__ out)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m)
+        return;
+    unsigned long long w[kExactWords];
+#pragma unroll
+    for (int k = 0; k < kExactWords; ++k)
+        w[k] = acc[(long long)k * m + j];
+    out[j] = exact_round(w, acc[(long long)kExactWords * m + j]);
+}
+
+hipError_t launch_exact_finish(const unsigned long long *acc, long long stride, long long n,
+                               float *img, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    hipLaunchKernelGGL(exact_finish_kernel, dim3((unsigned)blocks), dim3(256), 0, st, acc, stride, n, img);
+    return hipGetLastError();
+}
+
+hipError_t launch_exact_probe(const float *terms, long long n, int m, unsigned long long *acc,
+                              float *out, hipStream_t st)
+{
+    if (m <= 0)
+        return hipSuccess;
+    hipError_t e = launch_fill_f32(reinterpret_cast<float *>(acc), 2ll * kExactPlanes * m, 0.f, st);
+    if (e != hipSuccess)
+        return e;
+    if (n > 0) {
+        long long blocks = (n * m + 255) / 256;
+        if (blocks > 4096)
+            blocks = 4096;
+        hipLaunchKernelGGL(exact_probe_add_kernel, dim3((unsigned)blocks), dim3(256), 0, st, terms, n, m, acc);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    hipLaunchKernelGGL(exact_probe_round_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, acc, m, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f32(float *p, long long n, float v, hipStream_t st)
 {
     if (n <= 0)

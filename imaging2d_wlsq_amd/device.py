@@ -157,6 +157,19 @@ def trace_spectra_device(traces: torch.Tensor, out: Optional[torch.Tensor] = Non
     return out
 
 
+def exact_sum(terms: torch.Tensor, stream=None) -> torch.Tensor:
+    """(m,) float32: the float nearest the exact sum over rows of an (n, m)
+    float32 device tensor, through the fused kernel's exact image
+    accumulation (wlsq_probe_exact_sum; a measurement and test aid)."""
+    if terms.dim() != 2:
+        raise ValueError("terms must be (n, m)")
+    n, m = terms.shape
+    out = torch.empty(m, dtype=torch.float32, device=terms.device)
+    check(lib().wlsq_probe_exact_sum(_ptr(terms, torch.float32, "terms"), int(n), int(m),
+                                     out.data_ptr(), _stream(stream)))
+    return out
+
+
 def tables_conjugate(table_forw: torch.Tensor, table_back: torch.Tensor) -> bool:
     """True when table_back == conj(table_forw) bit for bit (then the fused
     kernel reads one table)."""

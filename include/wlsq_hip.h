@@ -192,6 +192,19 @@ typedef struct {
                               of the rows (two barriers per step). With
                               lds_budget, the autotuner uses it to trade rows
                               per task against tasks. */
+    int exact_image;      /* 1: each task's image sums (image modes 1 and 4)
+                             are added exactly, as 64-bit integers on the
+                             2^-149 grid into 16 words per image point (plus
+                             a flags word for infinities and NaN), and rounded
+                             once after the launch to the nearest float (ties
+                             to even): the image does not depend on the order
+                             of the atomic adds, so launches of one plan give
+                             it bit for bit, and it is the correctly rounded
+                             sum of the task sums. 136 bytes of plan memory
+                             per image point; plan launches only (the
+                             workspace and allocation-free entries refuse it),
+                             image modes 1 and 4, tables_conjugate = 1;
+                             2^24 task sums per point at most. */
 } wlsq_tuning;
 
 int wlsq_ktable_device(int ns, int nextrap, int nz, int nt, int nf, int nx, int M,
@@ -339,6 +352,13 @@ int wlsq_graph_destroy(wlsq_graph graph);
  * calibrates the VALU counters of a profile, and its time gives the FMA
  * issue rate the device reaches with packed FP32 disabled. */
 int wlsq_probe_fma(int blocks, int iters, float *out, void *stream);
+
+/* Measurement aid, no counterpart in the reference: out[j] = the float32
+ * nearest (ties to even) the exact sum over i < n of terms[i * m + j], the
+ * terms added by many threads in no fixed order through the exact image
+ * accumulation of the fused kernel (exact_image); device pointers, n <= 2^24;
+ * synchronous (allocates its accumulators). */
+int wlsq_probe_exact_sum(const float *terms, long long n, int m, float *out, void *stream);
 
 /* Points (x, z, omega, shot) the batched form computes for these shots. */
 long long wlsq_batch_points(const wlsq_shot *shots, int nshots);
