@@ -70,6 +70,19 @@ def candidates(conj: bool, sym: bool, max_nx: int) -> List[dict]:
     return out
 
 
+def longest_run(shots: Sequence[device.ShotSpec]) -> int:
+    """Most consecutive shots that can share a group (same window, depth
+    count, frequency list and strides, as the planner requires): candidates
+    with more shots per group would run groups of fewer shots."""
+    key = lambda s: (s.nx, s.nz, s.nf, s.omega_off, s.slow_off, s.slow_ld, s.ld, s.img_ld,  # noqa: E731
+                     s.first_step)
+    best = run = 1
+    for a, b in zip(shots, shots[1:]):
+        run = run + 1 if key(a) == key(b) else 1
+        best = max(best, run)
+    return best
+
+
 def _time_plan(plan: device.Plan, args, reps: int, stream) -> float:
     slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image = args
     plan.launch(slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image, stream=stream)
@@ -88,6 +101,7 @@ def tune(shots: Sequence[device.ShotSpec], M: int, slowness, omega, kappa0, dkap
          cands: Optional[Iterable[dict]] = None, stream=None, verbose=False) -> dict:
     """Best tuning dict for this batch (``image`` is used as scratch and
     left holding garbage; F/B are not written back)."""
+    cands_given = cands
     key = signature(shots, M, conj, sym)
     if key in _MEM:
         return dict(_MEM[key])
@@ -103,7 +117,9 @@ def tune(shots: Sequence[device.ShotSpec], M: int, slowness, omega, kappa0, dkap
     max_nx = max(s.nx for s in shots)
     results = []
     args = (slowness, omega, kappa0, dkappa, nk, tf, tb, F, B, image)
-    for c in (cands or candidates(conj, sym, max_nx)):
+    cands = list(cands) if cands is not None else [c for c in candidates(conj, sym, max_nx)
+                                                   if c["shots_per_group"] <= longest_run(shots)]
+    for c in cands:
         try:
             plan = device.Plan(shots, M, c)
         except WlsqError:
@@ -125,7 +141,7 @@ def tune(shots: Sequence[device.ShotSpec], M: int, slowness, omega, kappa0, dkap
     # depth steps per launch: all of them, or the range split in 2 or 4
     # launches (each restarts from the written-back fields)
     max_nz = max(s.nz for s in shots)
-    if cands is None and max_nz >= 64:
+    if cands_given is None and max_nz >= 64:
         for parts in (2, 4):
             c = dict(best, depth_steps=(max_nz + parts - 1) // parts)
             try:

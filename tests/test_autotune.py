@@ -65,3 +65,12 @@ def test_tuned_shape_gives_the_default_image_and_is_cached(tmp_path):
     assert again == first
     ref.close()
     b.close()
+
+
+def test_longest_run_counts_shots_that_can_share_a_group():
+    from imaging2d_wlsq_amd import autotune, device
+    mk = lambda nx, x0: device.ShotSpec(0, 0, 0, x0, nx, 1000, 1000, nx, 50, 10, 0)  # noqa: E731
+    assert autotune.longest_run([mk(100, 0)]) == 1
+    assert autotune.longest_run([mk(100, 0), mk(100, 0), mk(200, 0), mk(100, 0)]) == 2
+    assert autotune.longest_run([mk(100, 0), mk(100, 5)]) == 1          # other window
+    assert autotune.longest_run([mk(64, 0)] * 9) == 9
