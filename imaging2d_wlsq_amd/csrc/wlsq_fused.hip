@@ -194,8 +194,10 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     __syncthreads();
 
     float2 accF_[C][S][R], accB_[C][S][R];
-    float sv_[C][R];  // one strip per thread: slowness of the coming step at the strip's x
-    unsigned soff[R];  // one strip per thread: byte offsets of those x in a slowness row
+    // slowness of the coming step at each strip's x, loaded a step ahead
+    constexpr bool PREFETCH = C == 1 || CONJ;
+    float sv_[C][R];
+    unsigned soff[R];  // byte offsets of the first strip's x in a slowness row
 #pragma unroll
     for (int r = 0; r < R; ++r)
         soff[r] = (unsigned)min(x0v[0] + r, nx - 1) * 4u;
@@ -203,7 +205,9 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
     for (int cs = 0; cs < C; ++cs) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            sv_[cs][r] = (C == 1 && actv[cs]) ? at_bytes(slow0 + (long long)lb * slow_ld, soff[r]) : 0.f;
+            sv_[cs][r] = (PREFETCH && actv[cs]) ? at_bytes(slow0 + (long long)lb * slow_ld,
+                                             (unsigned)min(x0v[cs] + r, nx - 1) * 4u)
+                                  : 0.f;
     }
     // image sums from the rows: the slot of x = tid in a padded row
     const int fps0 = swz<R>(tid + M, Q);
@@ -232,13 +236,15 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
             float2 (&accB)[S][R] = accB_[cs];
             const int x0 = x0v[cs];
             const float om = omv[cs];
-            // One strip per thread: the slowness of this step was loaded during
-            // the previous one, and the next step's load is issued now so its
-            // latency hides behind the taps. Two strips per thread have no
-            // registers to spare and load at the start of the step.
+            // The slowness of this step was loaded during the previous one,
+            // and the next step's load is issued now so its latency hides
+            // behind the taps (at the start of a step, after the barrier,
+            // every wavefront would wait for it at once). Two strips per
+            // thread with separate backward tables have no registers to spare
+            // (10-19 more values spill) and load at the start of the step.
             // byte offsets of the strip's table rows
             unsigned qo[R];
-            if constexpr (C == 1) {
+            if constexpr (PREFETCH) {
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     qo[r] = (unsigned)__mul24(kappa_index_neg(om, sv_[cs][r], a.kappa0, a.inv_dk, a.nk), nLp8);
@@ -252,9 +258,17 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                     for (int r = 0; r < R; ++r) {
                         // opaque to loop-invariant code motion: the offset is
                         // widened here, next to the load, which then takes the
-                        // row as its scalar base (no 64-bit address per lane)
-                        asm volatile("" : "+v"(soff[r]));
-                        sv_[cs][r] = at_bytes(srow, soff[r]);
+                        // row as its scalar base (no 64-bit address per lane);
+                        // the second strip forms its offsets here each step
+                        unsigned o;
+                        if constexpr (cs == 0) {
+                            asm volatile("" : "+v"(soff[r]));
+                            o = soff[r];
+                        } else {
+                            o = (unsigned)min(x0 + r, nx - 1) * 4u;
+                            asm volatile("" : "+v"(o));
+                        }
+                        sv_[cs][r] = at_bytes(srow, o);
                     }
                 }
             } else {

@@ -651,7 +651,7 @@ def test_two_strips_per_thread_match_one(budget):
         print("tuning", t, b.plan.info(), flush=True)
         specs = b.specs
         b.close()
-    assert outs[1][3]["items"] <= outs[0][3]["items"]
+    assert outs[1][3]["points"] == outs[0][3]["points"]
     np.testing.assert_array_equal(outs[1][1], outs[0][1])
     np.testing.assert_array_equal(outs[1][2], outs[0][2])
     assert rel(outs[1][0], outs[0][0]) <= 1e-6
@@ -660,3 +660,52 @@ def test_two_strips_per_thread_match_one(budget):
         with pytest.raises(WlsqError) as e:
             device.Plan(specs, p.tables.M, t)
         assert e.value.status == code
+
+
+def test_balanced_task_rows_match_fixed_rows():
+    """1024-thread plans split a shot's frequencies into tasks that balance the
+    four SIMDs (1500-wide rows at two strips per thread: 23 frequencies as
+    4+4+4+4+4+3 instead of 5+5+5+5+3); omegas_per_item fixes the rows per task
+    instead. Both give the oracle's image, and the same final fields bit for
+    bit (a row's steps do not depend on its task)."""
+    import torch
+    from imaging2d_wlsq_amd import batch, synth
+    p = _small_problem(seed=22, nz=24, nx=1600)
+    shots = [synth.ShotGeom(50, 1500, 24, 750.0, 2.0 + 38.0 * np.arange(23) / 22, 2201),
+             synth.ShotGeom(300, 1200, 17, 600.0, 2.0 + 38.0 * np.arange(11) / 10, 2202),
+             synth.ShotGeom(0, 900, 24, 450.0, 2.0 + 38.0 * np.arange(17) / 16, 2203)]
+    sv = batch.Survey(p)
+    base = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym), x_per_thread=4,
+                shots_per_group=1, block=1024, strips_per_thread=2, lds_budget=120 << 10)
+    outs = []
+    for t in (base, dict(base, omegas_per_item=64)):
+        b = sv.batch(shots, tuning=t, write_back=False)
+        img = b.run().clone()
+        torch.cuda.synchronize()
+        info = b.plan.info()
+        print("tuning", t, info, flush=True)
+        F0, B0 = b.F.cpu().numpy(), b.B.cpu().numpy()
+        b.close()
+        b = sv.batch(shots, tuning=t, write_back=True)
+        b.run()
+        torch.cuda.synchronize()
+        outs.append((img.cpu().numpy(), b.F.cpu().numpy(), b.B.cpu().numpy(), info, F0, B0, b.specs))
+        b.close()
+    (img, F, B, info, F0, B0, specs), fixed = outs
+    assert info["points"] == fixed[3]["points"]
+    # the 1500-wide shot alone: 6 tasks instead of 5
+    assert info["items"] == fixed[3]["items"] + 1
+    np.testing.assert_array_equal(F, fixed[1])
+    np.testing.assert_array_equal(B, fixed[2])
+    assert rel(img, fixed[0]) <= 1e-6
+    m, tb = p.model, p.tables
+    ref = np.zeros((m.nz, m.nx))
+    for spec, s in zip(specs, shots):
+        Fs = F0[spec.f_off:spec.f_off + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
+        Bs = B0[spec.b_off:spec.b_off + s.nf * s.nx].reshape(1, s.nf, s.nx).copy()
+        io = np.zeros((1, s.nz, s.nx), np.float32)
+        sl = np.ascontiguousarray(m.slowness[:s.nz, s.x0:s.x0 + s.nx])
+        oracle.extrapolate_ktable(1, s.nz, s.nz, s.nx, s.nf, s.nf, tb.M, sl, s.omega, tb.kappa0,
+                                  tb.dkappa, tb.nk, tb.forw, tb.back, Fs, Bs, io)
+        ref[:s.nz, s.x0:s.x0 + s.nx] += io[0]
+    assert rel(img, ref) <= 1e-5
