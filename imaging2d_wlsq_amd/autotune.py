@@ -62,7 +62,7 @@ def candidates(conj: bool, sym: bool, max_nx: int) -> List[dict]:
                     # strips per thread (twice the rows per task, one copy of
                     # them) at LDS budgets near the device's
                     if wide and block == 1024 and strips <= 1024:
-                        for budget in (112 << 10, 120 << 10):
+                        for budget in (120 << 10, 160 << 10):
                             out.append(dict(shots_per_group=S, x_per_thread=R, block=block,
                                             omegas_per_item=0, tables_conjugate=int(conj),
                                             tables_symmetric=sy, tap_loads=2,
