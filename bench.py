@@ -359,7 +359,7 @@ def bench_serving(args, log):
         launcher.warm(lanes=args.serve_lanes)
         mk = lambda: serve.RoundServer(launcher, round_points=args.serve_round_points,  # noqa: E731
                                        max_active=args.serve_max_active, lanes=args.serve_lanes,
-                                       min_steps=args.serve_min_steps)
+                                       min_steps=args.serve_min_steps, stage=args.serve_stage)
     else:
         launcher = serve.GpuLauncher(models, args.serve_slots)
         mk = lambda: serve.Server(launcher, hbm_budget=budget)  # noqa: E731
@@ -388,6 +388,7 @@ def bench_serving(args, log):
                   "rounds_with_jobs_at_different_depths":
                       sum(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log),
                   "round_points": args.serve_round_points, "max_active": args.serve_max_active,
+                  "staged": args.serve_stage,
                   "lanes": args.serve_lanes, "min_steps": args.serve_min_steps,
                   "one_shot_tuning": launcher.one_shot_tuning})
     else:
@@ -440,6 +441,9 @@ def main():
                     help="points per round of --serve-mode rounds")
     ap.add_argument("--serve-max-active", type=int, default=8,
                     help="running jobs at most (--serve-mode rounds)")
+    ap.add_argument("--serve-stage", type=int, default=2,
+                    help="jobs admitted beyond --serve-max-active whose inputs are generated ahead "
+                         "(--serve-mode rounds)")
     ap.add_argument("--serve-lanes", type=int, default=4,
                     help="round sequences per cohort (--serve-mode rounds)")
     ap.add_argument("--serve-strips", type=int, default=2, choices=[1, 2],

@@ -333,6 +333,7 @@ class _RoundJob:
     t_admit: float
     z: int = 0             # next depth step
     done_round: int = -1   # round after which the job is complete
+    joined: bool = False   # takes part in rounds (its inputs are ready and a slot was free)
 
 
 @dataclass
@@ -352,11 +353,17 @@ class RoundServer:
     `steps` depth steps, chosen so a round holds about `round_points` points
     (between `min_steps` and `max_steps`, a multiple of `min_steps`). Arrived
     jobs are admitted first come first served while fewer than `max_active`
-    run and the engine can place their wavefields and image."""
+    + `stage` are admitted and the engine can place their wavefields and
+    image; an admitted job joins the rounds once its inputs are ready and
+    fewer than `max_active` jobs run. With `stage` > 0 the inputs of the next
+    jobs are generated while `max_active` jobs run, so a job that leaves is
+    replaced at once instead of after its successor's input generation."""
 
     def __init__(self, engine, round_points: float = 3e9, min_steps: int = 16,
-                 max_steps: int = 512, depth: int = 2, max_active: int = 8, lanes: int = 1):
+                 max_steps: int = 512, depth: int = 2, max_active: int = 8, lanes: int = 1,
+                 stage: int = 0):
         self.engine = engine
+        self.stage = max(0, int(stage))
         # jobs of one cohort are spread over `lanes` round sequences (the lane
         # with the least work left takes a new job); the lanes' rounds are
         # independent, so one lane's round can fill the tail of another's
@@ -390,10 +397,12 @@ class RoundServer:
         waiting: deque = deque()
         cohorts: Dict[tuple, List[_RoundJob]] = {}
         inflight: Dict[tuple, deque] = {}
-        n_active = 0
+        staged: deque = deque()   # admitted, not yet running (admission order)
+        n_admitted = 0            # running and staged jobs
+        n_running = 0
         eng.sync()
         t0 = clock()
-        while pending or waiting or n_active:
+        while pending or waiting or n_admitted:
             now = clock() - t0
             progressed = False
             while pending and pending[0].arrival / time_scale <= now:
@@ -415,12 +424,13 @@ class RoundServer:
                             st.images[r.job.job_id] = eng.image(r.state)
                         eng.release(r.state)
                         cohorts[key].remove(r)
-                        n_active -= 1
+                        n_admitted -= 1
+                        n_running -= 1
                         if log and len(st.latencies) % max(1, log_every) == 0:
                             log(f"job {r.job.job_id} done ({len(st.latencies)} of {len(jobs)}): "
                                 f"{r.job.points:.3g} pts, latency {st.latencies[-1]:.3f} s")
             # admission: first come first served, while the pools place the job
-            while waiting and n_active < self.max_active:
+            while waiting and n_admitted < self.max_active + self.stage:
                 job = waiting[0]
                 state = eng.admit(job)
                 if state is None:
@@ -432,17 +442,29 @@ class RoundServer:
                 left = lambda k: sum(r.job.nx * r.job.nf * r.job.nshots * (r.job.nz - r.z)  # noqa: E731
                                      for r in cohorts.get(k, []))
                 key = min((base + (ln,) for ln in range(self.lanes)), key=left)
-                cohorts.setdefault(key, []).append(_RoundJob(job, key, state, t_adm))
+                rj = _RoundJob(job, key, state, t_adm)
+                cohorts.setdefault(key, []).append(rj)
                 inflight.setdefault(key, deque())
-                n_active += 1
-                st.max_active = max(st.max_active, n_active)
+                staged.append(rj)
+                n_admitted += 1
                 st.peak_bytes = max(st.peak_bytes, eng.bytes_in_use())
                 progressed = True
+            # admitted jobs start running, in admission order among those whose
+            # inputs are ready, while a slot is free
+            for r in list(staged):
+                if n_running >= self.max_active:
+                    break
+                if eng.ready(r.state):
+                    staged.remove(r)
+                    r.joined = True
+                    n_running += 1
+                    st.max_active = max(st.max_active, n_running)
+                    progressed = True
             # rounds: every cohort keeps up to `depth` rounds queued
             for key, members in cohorts.items():
                 q = inflight[key]
                 while len(q) < self.depth:
-                    run = [r for r in members if r.z < r.job.nz and eng.ready(r.state)]
+                    run = [r for r in members if r.z < r.job.nz and r.joined]
                     if not run:
                         break
                     steps = self._steps(run)

@@ -600,8 +600,9 @@ def test_round_entry_refuses_bad_input():
 
 def test_round_server_same_alone_and_in_the_mix():
     """Continuous batching of a stream of small jobs on two models, at most 3
-    running: every job's image is the same alone and in the mix, jobs joined
-    rounds under way, and the pools are empty at the end."""
+    running (and 2 more admitted with their inputs generated ahead): every
+    job's image is the same alone and in the mix, jobs joined rounds under
+    way, and the pools are empty at the end."""
     from imaging2d_wlsq_amd import serve
     models = [_small_problem(31), _small_problem(32, nz=50, nx=700)]
     jobs = serve.make_jobs(12, 1e9, 2, 40, 600, seed=19, nx_range=(64, 600), nz_range=(8, 40),
@@ -613,17 +614,18 @@ def test_round_server_same_alone_and_in_the_mix():
             [j], keep_images=True).images[j.job_id]
     for j in jobs:
         j.arrival = 0.0
-    for lanes in (1, 2):
-        srv = serve.RoundServer(eng, round_points=1e5, min_steps=4, max_active=3, lanes=lanes)
+    for lanes, stage in ((1, 0), (2, 0), (1, 2)):
+        srv = serve.RoundServer(eng, round_points=1e5, min_steps=4, max_active=3, lanes=lanes,
+                                stage=stage)
         st = srv.run(jobs, keep_images=True)
-        print("lanes", lanes, "rounds", srv.rounds_launched, st.summary(), flush=True)
+        print("lanes", lanes, "stage", stage, "rounds", srv.rounds_launched, st.summary(), flush=True)
         assert st.max_active == 3
         if lanes == 1:
             assert any(len({z for _, z in m}) > 1 for _, m, _, _ in srv.round_log)
         else:
             assert len({k[-1] for k, _, _, _ in srv.round_log}) == 2
         for j in jobs:
-            assert rel(st.images[j.job_id], alone[j.job_id]) <= 1e-6, (lanes, j)
+            assert rel(st.images[j.job_id], alone[j.job_id]) <= 1e-6, (lanes, stage, j)
         assert eng.fpool.used == 0 and eng.ipool.used == 0
     eng.close()
 

@@ -128,10 +128,15 @@ class FakeRoundEngine:
     with `capacity` bytes of pools; records the depth range every round gave
     every job."""
 
-    def __init__(self, clock, rate=1e9, capacity=1 << 40):
+    def __init__(self, clock, rate=1e9, capacity=1 << 40, gen_time=0.0):
         self.clock = clock
         self.rate = rate
         self.capacity = capacity
+        # inputs of admitted jobs are generated one after another, gen_time each
+        self.gen_time = gen_time
+        self.t_gen = 0.0
+        self.held = 0        # admitted and not yet released
+        self.max_held = 0
         self.used = 0
         self.t_free = 0.0
         self.ranges = {}     # job id -> [(first step, end step), ...]
@@ -139,8 +144,8 @@ class FakeRoundEngine:
         self.admitted = []
 
     class S:
-        def __init__(self, job, nbytes):
-            self.job, self.nbytes = job, nbytes
+        def __init__(self, job, nbytes, t_ready=0.0):
+            self.job, self.nbytes, self.t_ready = job, nbytes, t_ready
 
     class H:
         def __init__(self, t_end):
@@ -160,7 +165,10 @@ class FakeRoundEngine:
             return None
         self.used += self.nbytes(job)
         self.admitted.append(job.job_id)
-        return FakeRoundEngine.S(job, self.nbytes(job))
+        self.held += 1
+        self.max_held = max(self.max_held, self.held)
+        self.t_gen = max(self.clock(), self.t_gen) + self.gen_time
+        return FakeRoundEngine.S(job, self.nbytes(job), self.t_gen)
 
     def launch_round(self, key, members, steps):
         pts = 0
@@ -182,13 +190,14 @@ class FakeRoundEngine:
         return self.clock() >= h.t_end
 
     def ready(self, s):
-        return True
+        return self.clock() >= s.t_ready
 
     def image(self, s):
         return np.full((2, 2), float(s.job.job_id))
 
     def release(self, s):
         self.used -= s.nbytes
+        self.held -= 1
 
     def bytes_in_use(self):
         return self.used
@@ -238,6 +247,34 @@ def test_jobs_join_running_rounds():
     assert st.order == [1, 0]
     mixed = [m for _, m, _ in eng.rounds if len(m) == 2]
     assert mixed and all(dict(m)[0] > dict(m)[1] for m in mixed)
+
+
+def test_staged_jobs_hide_input_generation():
+    """With `stage` > 0 the next jobs are admitted while `max_active` run and
+    their inputs are generated then, so a job that leaves is replaced at once:
+    the same jobs finish sooner, never more than `max_active` run, at most
+    `max_active` + `stage` are admitted, and every job's rounds still tile its
+    depth range."""
+    jobs = _jobs(n=30, rate=100.0)
+    for j in jobs:
+        j.arrival = 0.0
+    walls, held = {}, {}
+    for stage in (0, 2):
+        clock = FakeClock()
+        mean_run = np.mean([j.points for j in jobs]) / 1e9
+        eng = FakeRoundEngine(clock, rate=1e9, gen_time=0.5 * mean_run)
+        srv = serve.RoundServer(eng, round_points=2e6, min_steps=4, max_steps=32, max_active=3,
+                                stage=stage)
+        st = srv.run(jobs, clock=clock, idle=clock.idle)
+        assert sorted(st.order) == [j.job_id for j in jobs] and eng.used == 0
+        assert st.max_active == 3
+        for j in jobs:
+            r = eng.ranges[j.job_id]
+            assert r[0][0] == 0 and r[-1][1] == j.nz
+            assert all(a[1] == b[0] for a, b in zip(r, r[1:]))
+        walls[stage], held[stage] = st.wall, eng.max_held
+    assert held[0] == 3 and held[2] == 5
+    assert walls[2] < 0.95 * walls[0], walls
 
 
 def test_round_steps_follow_the_round_size():
