@@ -544,6 +544,9 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
     if (rows_img)
         img_lds = 2;
     std::vector<Pending> pend;
+    // the task rows of each (frequencies, rows per task, strips) the plan meets
+    // (a batch of many shots of one shape solves the split once)
+    std::map<std::tuple<int, int, int>, std::vector<int>> rows_memo;
     size_t lds_need = 0;
     for (auto &rn : runs) {
         const wlsq_shot &s = shots[rn.first];
@@ -583,9 +586,15 @@ int build_plan(Device *d, const wlsq_shot *shots, int nshots, int M, const wlsq_
         lds_need = std::max(lds_need, need);
         const int gi = (int)P->groups.size();
         P->groups.push_back(g);
+        auto memo = rows_memo.find(std::make_tuple(s.nf, W, strips));
+        if (memo == rows_memo.end())
+            memo = rows_memo
+                       .emplace(std::make_tuple(s.nf, W, strips),
+                                task_rows(s.nf, W, strips, block, C,
+                                          t.omegas_per_item == 0 && block == 1024))
+                       .first;
         int j0 = 0;
-        for (const int nw : task_rows(s.nf, W, strips, block, C,
-                                      t.omegas_per_item == 0 && block == 1024)) {
+        for (const int nw : memo->second) {
             Pending pd;
             pd.item.group = gi;
             pd.item.j0 = j0;
