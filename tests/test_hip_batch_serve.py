@@ -283,7 +283,8 @@ _EXTREME_REFS = {}
 
 def _served(kind, models, jobs):
     """Each job served alone, then all of them at once (max 4 running), by the
-    per-job stream launcher or by the continuous-batching rounds."""
+    per-job stream launcher or by the continuous-batching rounds (with 2 more
+    jobs staged, as bench.py serves)."""
     from imaging2d_wlsq_amd import serve
     if kind == "streams":
         launcher = serve.GpuLauncher(models, slots=4)
@@ -291,7 +292,8 @@ def _served(kind, models, jobs):
     else:
         launcher = serve.GpuRoundEngine(models, hbm_budget=8 << 30)
         launcher.warm(lanes=2)
-        mk = lambda: serve.RoundServer(launcher, round_points=2e8, max_active=4, lanes=2)  # noqa: E731
+        mk = lambda: serve.RoundServer(launcher, round_points=2e8, max_active=4, lanes=2,  # noqa: E731
+                                       stage=2)
     alone = {}
     for j in jobs:
         st = mk().run([j], keep_images=True)
