@@ -632,19 +632,22 @@ def test_round_server_same_alone_and_in_the_mix():
     eng.close()
 
 
+@pytest.mark.parametrize("conj", [True, False])
 @pytest.mark.parametrize("budget", [0, 120 << 10])
-def test_two_strips_per_thread_match_one(budget):
+def test_two_strips_per_thread_match_one(budget, conj):
     """strips_per_thread = 2 on apertures one strip per thread covers (twice
     the frequency rows per task, one copy of the rows): the final fields of
     the one-strip shape bit for bit and its image to the rounding of the
-    atomic order; invalid requests are refused."""
+    atomic order, with one conjugate table (the next step's slowness loaded
+    during the current one) and with separate backward tables (loaded at the
+    start of the step); invalid requests are refused."""
     import torch
     from imaging2d_wlsq_amd import batch, device
     from imaging2d_wlsq_amd._lib import WLSQ_EINVAL, WLSQ_EUNSUPPORTED, WlsqError
     p = _small_problem(seed=20, nz=30, nx=900)
     shots = _uneven_shots(p, 6, 91)
     sv = batch.Survey(p)
-    base = dict(tables_conjugate=int(sv.conj), tables_symmetric=int(sv.sym), x_per_thread=4,
+    base = dict(tables_conjugate=int(sv.conj and conj), tables_symmetric=int(sv.sym), x_per_thread=4,
                 shots_per_group=1, block=1024)
     outs = []
     for t in (base, dict(base, strips_per_thread=2, lds_budget=budget)):
