@@ -107,7 +107,13 @@ int extrapolate_ktable(int ns, int nextrap, int nz, int nt, int nf, int nx, int 
 
 typedef struct {
     int shots_per_group; /* 0 = auto, else 1, 2 or 4 shots sharing coefficient loads */
-    int omegas_per_item; /* 0 = auto: frequencies per workgroup task */
+    int omegas_per_item; /* 0 = auto: frequencies per workgroup task, at most
+                            what the threads and the LDS budget allow; with
+                            1024 threads each shot's frequencies are split
+                            into tasks that balance the four SIMDs (so the
+                            tasks of one shot may differ by a few rows);
+                            n > 0 = at most n, and every task of a shot but
+                            the last takes the same count */
     int block;           /* 0 = auto: threads per workgroup, a multiple of 64:
                             up to 1024 for one shot per group at 4 x per
                             thread and for two shots at 2 x, else up to 512 */
