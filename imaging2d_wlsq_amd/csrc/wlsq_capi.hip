@@ -334,9 +334,10 @@ long long shot_steps(const wlsq_shot &s, int round_steps)
 // task for its flush and image atomics. Example: 1500-wide rows (375 strips of
 // 4 x) at C = 2 and W = 5 fill waves 0-15 and 0-13 (busiest SIMD 8 passes,
 // 5 rows), where tasks of 4 rows fill 0-15 and 0-7 (6 passes, 4 rows): per row
-// 1.6 against 1.5 passes. Measured on MI355X the split is worth 0.3 % on a
-// batch of 128 shots of apertures 256-2048: the step time follows the
-// wavefronts each SIMD keeps active (latency hiding) more than its pass count.
+// 1.6 against 1.5 passes. On MI355X, batches of one aperture at fixed rows per
+// task give a step time of about a * (busiest SIMD's passes) + b with b near a
+// (1500-wide rows: 7 % faster in tasks of 4 rows than of 5); on a batch of 128
+// shots of apertures 256-2048 the split is worth 0.3 %.
 std::vector<int> task_rows(int nf, int W, int strips, int block, int C, bool balance)
 {
     std::vector<int> out;
