@@ -209,8 +209,6 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
                                              (unsigned)min(x0v[cs] + r, nx - 1) * 4u)
                                   : 0.f;
     }
-    // image sums from the rows: the slot of x = tid in a padded row
-    const int fps0 = swz<R>(tid + M, Q);
     for (int l = lb; l < le; ++l) {
         const int cur = nbuf == 2 ? ((l - lb) & 1) : 0;
         const float4 *rows_r = rows + cur * rows_size;               // read this step
@@ -728,36 +726,49 @@ __global__ void __launch_bounds__(MAXT) fused_kernel(const FusedArgs a)
         // slot of x + NT is that of x plus NT / R (NT is a multiple of R).
         auto flush_rows = [&](const float4 *src, int ld_l) {
             float *dst = imgf_rows + (long long)ld_l * img_ld;
-            const float4 *e0 = src + fps0;
+            // the slot of x = tid, formed each step from an opaque copy of tid:
+            // a few integer operations instead of a value kept through the taps,
+            // where it would be spilled and reloaded with a memory wait per step
+            // (the slot is opaque too, so the rows' offsets stay additions)
+            int t = tid;
+            asm volatile("" : "+v"(t));
+            int slot = __mul24((t + M) & (R - 1), Q) + ((t + M) >> LOG2R);
+            asm volatile("" : "+v"(slot));
+            const float4 *e0 = src + slot;
             const unsigned xend = (unsigned)nx * 4u;
+            // the first row's term, and the sum so far plus a further row's
+            auto first = [&](const float4 g) {
+                if constexpr (EXACT)
+                    return rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                else
+                    return fmaf(g.y, g.w, g.x * g.z);
+            };
+            auto more = [&](float v, const float4 g) {
+                if constexpr (EXACT)
+                    return v + rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
+                else
+                    return fmaf(g.y, g.w, fmaf(g.x, g.z, v));
+            };
             if (W == 1) {
                 // single-frequency tasks (the wide apertures): one read per x
-                for (unsigned xo = (unsigned)tid * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
-                    const float4 g = *e0;
-                    float v;
-                    if constexpr (EXACT)
-                        v = rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
-                    else
-                        v = fmaf(g.y, g.w, g.x * g.z);
-                    img_add(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
-                }
+                for (unsigned xo = (unsigned)t * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R)
+                    img_add(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), first(*e0));
                 return;
             }
-            for (unsigned xo = (unsigned)tid * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
+            for (unsigned xo = (unsigned)t * 4u; xo < xend; xo += 4u * NT, e0 += NT >> LOG2R) {
                 const float4 *e = e0;
-                float4 g = *e;
-                float v;
-                if constexpr (EXACT)
-                    v = rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
-                else
-                    v = fmaf(g.y, g.w, g.x * g.z);
-                for (int w = 1; w < W; ++w) {
+                float v = first(*e);
+                // rows in order; four reads issued before their sums, so the
+                // LDS latency is paid once per four rows
+                int w = 1;
+                for (; w + 4 <= W; w += 4, e += 4 * rowlen) {
+                    const float4 g0 = e[rowlen], g1 = e[2 * rowlen], g2 = e[3 * rowlen],
+                                 g3 = e[4 * rowlen];
+                    v = more(more(more(more(v, g0), g1), g2), g3);
+                }
+                for (; w < W; ++w) {
                     e += rowlen;
-                    g = *e;
-                    if constexpr (EXACT)
-                        v += rmc(make_float2(g.x, g.y), make_float2(g.z, g.w));
-                    else
-                        v = fmaf(g.y, g.w, fmaf(g.x, g.z, v));
+                    v = more(v, *e);
                 }
                 img_add(reinterpret_cast<float *>(reinterpret_cast<char *>(dst) + xo), v);
             }
